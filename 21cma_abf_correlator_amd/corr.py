@@ -1,0 +1,365 @@
+"""Host-side binding of libcorr_cuda (the MI355X build) through its C ABI.
+
+`Correlator` mirrors the reference's safe Rust wrapper (corr-rs/src/gpu.rs:32-333)
+over the same 20 calls its FFI binds (corr-rs/src/ffi.rs:157-219), plus the
+asynchronous halves the C tests use (corr-cuda/include/corr_cuda.h:399-520).
+`Engine` binds the engine-level entry points of include/corrx.h, which take
+device pointers (integers, e.g. torch's data_ptr()) and a HIP stream handle.
+
+There is no CPU fallback: if the library is missing this module raises.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libcorr_cuda.so")
+
+ABI_VERSION = 7
+MAX_DEVICES = 8
+SPLIT_BASELINE_RANGE, SPLIT_PAIR_COVERING, SPLIT_TIME_SLICE = 0, 1, 2
+FORM_FP32, FORM_FP16, FORM_INT8 = 0, 1, 2
+
+STATUS_NAMES = {
+    0: "CORR_OK", 1: "CORR_ERR_ARG", 2: "CORR_ERR_ABI", 3: "CORR_ERR_CUDA",
+    4: "CORR_ERR_CUFFT", 5: "CORR_ERR_STATE", 6: "CORR_ERR_BUSY", 7: "CORR_ERR_NOMEM",
+    8: "CORR_ERR_NOT_IMPL", 9: "CORR_ERR_INTERNAL",
+}
+OK, ERR_ARG, ERR_ABI, ERR_CUDA, ERR_CUFFT, ERR_STATE, ERR_BUSY, ERR_NOMEM, ERR_NOT_IMPL, \
+    ERR_INTERNAL = range(10)
+
+
+class corr_config(ctypes.Structure):
+    _fields_ = [
+        ("abi_version", ctypes.c_uint32),
+        ("struct_size", ctypes.c_uint32),
+        ("device_id", ctypes.c_int32),
+        ("n_station", ctypes.c_int32),
+        ("n_freq", ctypes.c_int32),
+        ("frames_per_slot", ctypes.c_int32),
+        ("chunk_frames", ctypes.c_int32),
+        ("block_slots", ctypes.c_int32),
+        ("rows_per_fetch", ctypes.c_int32),
+        ("n_devices", ctypes.c_int32),
+        ("devices", ctypes.c_int32 * MAX_DEVICES),
+        ("split_mode", ctypes.c_int32),
+    ]
+
+
+class corr_device_plan(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in
+                ("device_id", "n_station_resident", "first_baseline", "n_baseline", "contiguous")]
+
+
+class corr_device_stats(ctypes.Structure):
+    _fields_ = [("device_id", ctypes.c_int32)] + [
+        (n, ctypes.c_uint64) for n in ("bytes_h2d", "bytes_d2h", "transfers", "ns_accumulate",
+                                       "ns_slot_free", "ns_row_end", "ns_rows_wait", "ns_fetch")]
+
+
+class corr_staging(ctypes.Structure):
+    _fields_ = [
+        ("base", ctypes.POINTER(ctypes.c_int8)),
+        ("frame_stride", ctypes.c_size_t),
+        ("slot_stride", ctypes.c_size_t),
+        ("total_bytes", ctypes.c_size_t),
+        ("validity_offset", ctypes.c_size_t),
+        ("validity_bytes_per_slot", ctypes.c_size_t),
+    ]
+
+
+class corrx_shape(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in ("n_station", "n_freq", "chunk_frames", "form")]
+
+
+NOTIFY_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p)
+
+# Every symbol include/corr_cuda.h and include/corrx.h declare.
+ABI_SYMBOLS = (
+    "corr_abi_banner", "corr_last_error", "corr_create", "corr_destroy", "corr_staging_info",
+    "corr_row_accumulate", "corr_row_slot_free", "corr_row_slot_free_submit",
+    "corr_row_slot_free_finish", "corr_row_end", "corr_rows_wait", "corr_rows_wait_submit",
+    "corr_rows_ready", "corr_rows_pending", "corr_rows_fetch", "corr_rows_fetch_submit",
+    "corr_rows_fetch_ready", "corr_n_baseline", "corr_baseline_pairs", "corr_n_devices",
+    "corr_device_plan_get", "corr_device_stations", "corr_device_baselines",
+    "corr_device_stats_get", "corr_host_alloc", "corr_host_free",
+)
+ENGINE_SYMBOLS = (
+    "corrx_engine_create", "corrx_engine_destroy", "corrx_fengine", "corrx_xengine",
+    "corrx_scale", "corrx_spectra",
+)
+
+_lib = None
+
+
+def load():
+    """Load libcorr_cuda.so and declare every signature. Raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i32, i64, u64, sz = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64, \
+        ctypes.c_size_t
+    st = ctypes.c_int
+    sig = {
+        "corr_abi_banner": ([], ctypes.c_char_p),
+        "corr_last_error": ([], ctypes.c_char_p),
+        "corr_create": ([ctypes.POINTER(corr_config), ctypes.POINTER(vp)], st),
+        "corr_destroy": ([vp], st),
+        "corr_staging_info": ([vp, i32, ctypes.POINTER(corr_staging)], st),
+        "corr_row_accumulate": ([vp, i64, i32], st),
+        "corr_row_slot_free": ([vp, i64], st),
+        "corr_row_slot_free_submit": ([vp, i64, NOTIFY_FN, vp], st),
+        "corr_row_slot_free_finish": ([vp, i64], st),
+        "corr_row_end": ([vp, u64], st),
+        "corr_rows_wait": ([vp], st),
+        "corr_rows_wait_submit": ([vp, NOTIFY_FN, vp], st),
+        "corr_rows_ready": ([vp], st),
+        "corr_rows_pending": ([vp, ctypes.POINTER(i32)], st),
+        "corr_rows_fetch": ([vp, vp, sz, ctypes.POINTER(i32)], st),
+        "corr_rows_fetch_submit": ([vp, vp, sz, ctypes.POINTER(i32), NOTIFY_FN, vp], st),
+        "corr_rows_fetch_ready": ([vp], st),
+        "corr_n_baseline": ([i32], i32),
+        "corr_baseline_pairs": ([i32, ctypes.POINTER(i32), i32], st),
+        "corr_n_devices": ([vp], i32),
+        "corr_device_plan_get": ([vp, i32, ctypes.POINTER(corr_device_plan)], st),
+        "corr_device_stations": ([vp, i32, ctypes.POINTER(i32), i32], st),
+        "corr_device_baselines": ([vp, i32, ctypes.POINTER(i32), i32], st),
+        "corr_device_stats_get": ([vp, i32, ctypes.POINTER(corr_device_stats)], st),
+        "corr_host_alloc": ([sz, ctypes.POINTER(vp)], st),
+        "corr_host_free": ([vp], None),
+        "corrx_engine_create": ([ctypes.POINTER(corrx_shape), ctypes.POINTER(vp)], st),
+        "corrx_engine_destroy": ([vp], st),
+        "corrx_fengine": ([vp, vp, sz, vp, sz, i32, i32, vp], st),
+        "corrx_xengine": ([vp, vp, i32, vp], st),
+        "corrx_scale": ([vp, sz, u64, vp], st),
+        "corrx_spectra": ([vp, ctypes.POINTER(vp), ctypes.POINTER(sz)], st),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+class CorrError(RuntimeError):
+    def __init__(self, status, what):
+        self.status = status
+        msg = load().corr_last_error().decode(errors="replace")
+        super().__init__(f"{what}: {STATUS_NAMES.get(status, status)}: {msg}")
+
+
+def check(status, what):
+    if status != OK:
+        raise CorrError(status, what)
+
+
+def abi_banner():
+    return load().corr_abi_banner().decode()
+
+
+def last_error():
+    return load().corr_last_error().decode(errors="replace")
+
+
+def n_baseline(n_station):
+    return load().corr_n_baseline(n_station)
+
+
+def baseline_pairs(n_station):
+    """[(j, k)] in the library's numbering (reference corr_cuda.cpp:1515-1534)."""
+    B = n_baseline(n_station)
+    if B < 0:
+        raise ValueError("n_station must be >= 1")
+    buf = (ctypes.c_int32 * (2 * B))()
+    check(load().corr_baseline_pairs(n_station, buf, 2 * B), "corr_baseline_pairs")
+    return [(buf[2 * i], buf[2 * i + 1]) for i in range(B)]
+
+
+def make_config(n_station, n_freq, frames_per_slot, chunk_frames, block_slots=3,
+                rows_per_fetch=1, device_id=0, devices=None, split_mode=SPLIT_BASELINE_RANGE):
+    c = corr_config()
+    c.abi_version = ABI_VERSION
+    c.struct_size = ctypes.sizeof(corr_config)
+    c.device_id = device_id
+    c.n_station = n_station
+    c.n_freq = n_freq
+    c.frames_per_slot = frames_per_slot
+    c.chunk_frames = chunk_frames
+    c.block_slots = block_slots
+    c.rows_per_fetch = rows_per_fetch
+    c.split_mode = split_mode
+    if devices:
+        c.n_devices = len(devices)
+        for i, d in enumerate(devices):
+            c.devices[i] = d
+    return c
+
+
+def create_raw(cfg):
+    """corr_create; returns (status, handle or None) without raising."""
+    h = ctypes.c_void_p()
+    s = load().corr_create(ctypes.byref(cfg), ctypes.byref(h))
+    return s, (h.value if h.value else None)
+
+
+class Correlator:
+    """One correlator context (reference corr-rs/src/gpu.rs:32-333)."""
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+        s, h = create_raw(cfg)
+        check(s, "corr_create")
+        self.h = h
+        self._lib = load()
+        self._staging = [self.staging(i) for i in range(cfg.n_station)]
+
+    def close(self):
+        if self.h is not None:
+            s = self._lib.corr_destroy(self.h)
+            self.h = None
+            check(s, "corr_destroy")
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- geometry -----------------------------------------------------------
+    @property
+    def n_station(self):
+        return self.cfg.n_station
+
+    @property
+    def n_freq(self):
+        return self.cfg.n_freq
+
+    @property
+    def n_baseline(self):
+        return n_baseline(self.cfg.n_station)
+
+    def row_bytes(self):
+        return self.n_baseline * self.cfg.n_freq * 8
+
+    def fetch_bytes(self):
+        return self.row_bytes() * self.cfg.rows_per_fetch
+
+    def staging(self, station):
+        out = corr_staging()
+        check(self._lib.corr_staging_info(self.h, station, ctypes.byref(out)), "corr_staging_info")
+        return out
+
+    def station_view(self, station):
+        """The station's whole staging extent as a writable int8 numpy array."""
+        s = self._staging[station]
+        return np.ctypeslib.as_array(s.base, shape=(s.total_bytes,))
+
+    def slot_frames(self, station, slot):
+        """(frames_per_slot, 2*n_freq) int8 view of one staging slot."""
+        s = self._staging[station]
+        v = self.station_view(station)
+        a = v[slot * s.slot_stride: (slot + 1) * s.slot_stride]
+        return a.reshape(self.cfg.frames_per_slot, s.frame_stride)
+
+    def slot_mask(self, station, slot):
+        s = self._staging[station]
+        v = self.station_view(station).view(np.uint8)
+        o = s.validity_offset + slot * s.validity_bytes_per_slot
+        return v[o: o + s.validity_bytes_per_slot]
+
+    # --- row lifecycle ------------------------------------------------------
+    def row_accumulate(self, slot_seq, frames):
+        check(self._lib.corr_row_accumulate(self.h, slot_seq, frames), "corr_row_accumulate")
+
+    def row_slot_free(self, slot_seq):
+        check(self._lib.corr_row_slot_free(self.h, slot_seq), "corr_row_slot_free")
+
+    def row_end(self, frames):
+        check(self._lib.corr_row_end(self.h, frames), "corr_row_end")
+
+    def rows_wait(self):
+        check(self._lib.corr_rows_wait(self.h), "corr_rows_wait")
+
+    def rows_pending(self):
+        p = ctypes.c_int32(-1)
+        check(self._lib.corr_rows_pending(self.h, ctypes.byref(p)), "corr_rows_pending")
+        return p.value
+
+    def rows_fetch(self, out=None):
+        """Blocking fetch into `out` (complex64 array of fetch_bytes()); returns (rows, out)."""
+        if out is None:
+            out = np.empty(self.fetch_bytes() // 8, dtype=np.complex64)
+        rows = ctypes.c_int32(-1)
+        check(self._lib.corr_rows_fetch(self.h, out.ctypes.data, out.nbytes, ctypes.byref(rows)),
+              "corr_rows_fetch")
+        return rows.value, out
+
+    def device_stats(self, device_index=0):
+        s = corr_device_stats()
+        check(self._lib.corr_device_stats_get(self.h, device_index, ctypes.byref(s)),
+              "corr_device_stats_get")
+        return s
+
+    def placement(self):
+        n = self._lib.corr_n_devices(self.h)
+        plans = []
+        for d in range(n):
+            p = corr_device_plan()
+            check(self._lib.corr_device_plan_get(self.h, d, ctypes.byref(p)), "corr_device_plan_get")
+            plans.append(p)
+        return plans
+
+
+class Engine:
+    """Engine-level stages on device buffers (include/corrx.h)."""
+
+    def __init__(self, n_station, n_freq, chunk_frames, form=FORM_FP32):
+        self._lib = load()
+        sh = corrx_shape(n_station, n_freq, chunk_frames, form)
+        h = ctypes.c_void_p()
+        check(self._lib.corrx_engine_create(ctypes.byref(sh), ctypes.byref(h)),
+              "corrx_engine_create")
+        self.h = h.value
+        self.n_station, self.n_freq, self.chunk_frames, self.form = \
+            n_station, n_freq, chunk_frames, form
+
+    def close(self):
+        if self.h is not None:
+            self._lib.corrx_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def fengine(self, d_in, station_pitch, frames, d_mask=None, mask_pitch=0, mask_bit0=0,
+                stream=None):
+        check(self._lib.corrx_fengine(self.h, d_in, station_pitch, d_mask, mask_pitch, mask_bit0,
+                                      frames, stream), "corrx_fengine")
+
+    def xengine(self, d_acc, frames, stream=None):
+        check(self._lib.corrx_xengine(self.h, d_acc, frames, stream), "corrx_xengine")
+
+    def spectra(self):
+        p = ctypes.c_void_p()
+        s = ctypes.c_size_t()
+        check(self._lib.corrx_spectra(self.h, ctypes.byref(p), ctypes.byref(s)), "corrx_spectra")
+        return p.value, s.value
+
+
+def scale(d_acc, n_complex, frames, stream=None):
+    check(load().corrx_scale(d_acc, n_complex, frames, stream), "corrx_scale")

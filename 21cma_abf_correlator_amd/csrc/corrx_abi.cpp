@@ -1,0 +1,61 @@
+// corrx_abi.cpp -- C entry points of the device engine (include/corrx.h).
+#include <hip/hip_runtime.h>
+
+#include "corrx.h"
+#include "engine.hpp"
+
+using corr::Engine;
+using corr::fail;
+
+static Engine *as_engine(void *e) { return static_cast<Engine *>(e); }
+static hipStream_t as_stream(void *s) { return static_cast<hipStream_t>(s); }
+
+extern "C" corr_status corrx_engine_create(const corrx_shape *shape, void **out_engine) {
+    if (out_engine == nullptr) return fail(CORR_ERR_ARG, "out_engine is NULL");
+    *out_engine = nullptr;
+    if (shape == nullptr) return fail(CORR_ERR_ARG, "shape is NULL");
+    if (shape->form < CORRX_FORM_FP32 || shape->form > CORRX_FORM_INT8)
+        return corr::failf(CORR_ERR_ARG, "form=%d is not a CORRX_FORM_*", shape->form);
+    corr::EngineShape s;
+    s.n_station = shape->n_station;
+    s.n_freq = shape->n_freq;
+    s.chunk_frames = shape->chunk_frames;
+    s.form = static_cast<corr::Form>(shape->form);
+    Engine *e = nullptr;
+    corr_status st = Engine::create(s, &e);
+    if (st != CORR_OK) return st;
+    *out_engine = e;
+    return CORR_OK;
+}
+
+extern "C" corr_status corrx_engine_destroy(void *engine) {
+    delete as_engine(engine);
+    return CORR_OK;
+}
+
+extern "C" corr_status corrx_fengine(void *engine, const int8_t *d_in, size_t station_pitch,
+                                     const uint8_t *d_mask, size_t mask_pitch, int32_t mask_bit0,
+                                     int32_t frames, void *stream) {
+    if (engine == nullptr) return fail(CORR_ERR_ARG, "engine is NULL");
+    return as_engine(engine)->fengine(d_in, station_pitch, d_mask, mask_pitch, mask_bit0, frames,
+                                      as_stream(stream));
+}
+
+extern "C" corr_status corrx_xengine(void *engine, void *d_acc, int32_t frames, void *stream) {
+    if (engine == nullptr) return fail(CORR_ERR_ARG, "engine is NULL");
+    return as_engine(engine)->xengine_fp32(static_cast<float2 *>(d_acc), frames, as_stream(stream));
+}
+
+extern "C" corr_status corrx_scale(void *d_acc, size_t n_complex, uint64_t frames, void *stream) {
+    if (d_acc == nullptr) return fail(CORR_ERR_ARG, "d_acc is NULL");
+    return Engine::scale(static_cast<float2 *>(d_acc), n_complex, frames, as_stream(stream));
+}
+
+extern "C" corr_status corrx_spectra(void *engine, void **d_spectra, size_t *station_stride_elems) {
+    if (engine == nullptr) return fail(CORR_ERR_ARG, "engine is NULL");
+    if (d_spectra == nullptr || station_stride_elems == nullptr)
+        return fail(CORR_ERR_ARG, "output pointer is NULL");
+    *d_spectra = as_engine(engine)->spectra();
+    *station_stride_elems = as_engine(engine)->spectra_station_stride();
+    return CORR_OK;
+}

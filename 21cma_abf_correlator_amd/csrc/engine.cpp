@@ -1,0 +1,158 @@
+// engine.cpp -- host side of the device engine: tables, plans, stage launches.
+#include "engine.hpp"
+
+#include <cstdarg>
+#include <cstdio>
+#include <new>
+
+namespace corr {
+
+std::string &last_error() {
+    static thread_local std::string s;
+    return s;
+}
+
+corr_status fail(corr_status st, const std::string &msg) {
+    last_error() = msg;
+    return st;
+}
+
+corr_status failf(corr_status st, const char *fmt, ...) {
+    char buf[768];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    last_error() = buf;
+    return st;
+}
+
+corr_status hip_check(hipError_t e, const char *what) {
+    if (e == hipSuccess) return CORR_OK;
+    return failf(CORR_ERR_CUDA, "%s: %s: %s", what, hipGetErrorName(e), hipGetErrorString(e));
+}
+
+bool fft_size_supported(int n_freq) {
+    if (n_freq < 512 || n_freq > 16384) return false;
+    return (n_freq & (n_freq - 1)) == 0;
+}
+
+static int ilog2(int v) {
+    int l = 0;
+    while ((1 << l) < v) ++l;
+    return l;
+}
+
+corr_status Engine::create(const EngineShape &shape, Engine **out) {
+    if (out == nullptr) return fail(CORR_ERR_ARG, "engine: out is NULL");
+    *out = nullptr;
+    if (shape.n_station < 1) return fail(CORR_ERR_ARG, "engine: n_station must be >= 1");
+    if (shape.chunk_frames < 1) return fail(CORR_ERR_ARG, "engine: chunk_frames must be >= 1");
+    if (!fft_size_supported(shape.n_freq))
+        return failf(CORR_ERR_NOT_IMPL,
+                     "n_freq=%d: this build transforms power-of-two n_freq in [512, 16384] "
+                     "only (FFT length 2*n_freq)",
+                     shape.n_freq);
+    if (shape.form != Form::kFP32)
+        return fail(CORR_ERR_NOT_IMPL, "engine: only the FP32 form is available in this build");
+    Engine *e = new (std::nothrow) Engine();
+    if (e == nullptr) return fail(CORR_ERR_NOMEM, "engine: out of host memory");
+    e->shape_ = shape;
+    corr_status st = e->init();
+    if (st != CORR_OK) {
+        delete e;
+        return st;
+    }
+    *out = e;
+    return CORR_OK;
+}
+
+corr_status Engine::init() {
+    const int N = shape_.n_station;
+    const int nf = shape_.n_freq;
+    n_baseline_ = N * (N + 1) / 2;
+    log2_nf_ = ilog2(nf);
+
+    corr_status st;
+    if ((st = hip_check(hipMalloc(reinterpret_cast<void **>(&d_tw_), sizeof(float2) * nf),
+                        "hipMalloc(twiddles)")) != CORR_OK ||
+        (st = hip_check(hipMalloc(reinterpret_cast<void **>(&d_wl_), sizeof(float2) * nf),
+                        "hipMalloc(r2c twiddles)")) != CORR_OK)
+        return st;
+    const size_t spec_bytes =
+        (size_t)N * (size_t)shape_.chunk_frames * (size_t)nf * sizeof(float2);
+    if ((st = hip_check(hipMalloc(&d_spec_, spec_bytes), "hipMalloc(spectra)")) != CORR_OK)
+        return st;
+
+    // X-engine plan: stations in blocks of 4, block pairs (jb <= kb) row major.
+    const int nb = (N + 3) / 4;
+    std::vector<int2> bp;
+    for (int jb = 0; jb < nb; ++jb)
+        for (int kb = jb; kb < nb; ++kb) bp.push_back(make_int2(jb, kb));
+    n_bpairs_ = (int)bp.size();
+    x_npair_ = n_bpairs_ > 16 ? 2 : 1;
+    const int slots = (n_bpairs_ + x_npair_ - 1) / x_npair_;
+    x_waves_ = slots < 16 ? slots : 16;
+    x_groups_ = (slots + x_waves_ - 1) / x_waves_;
+    const int npad = nb * 4;
+    x_frames_lds_ = 49152 / (npad * 64 * (int)sizeof(float2));
+    if (x_frames_lds_ < 1) x_frames_lds_ = 1;
+    if (x_frames_lds_ > 8) x_frames_lds_ = 8;
+    if ((st = hip_check(hipMalloc(reinterpret_cast<void **>(&d_bpairs_), sizeof(int2) * bp.size()),
+                        "hipMalloc(block pairs)")) != CORR_OK)
+        return st;
+    if ((st = hip_check(hipMemcpy(d_bpairs_, bp.data(), sizeof(int2) * bp.size(),
+                                  hipMemcpyHostToDevice),
+                        "hipMemcpy(block pairs)")) != CORR_OK)
+        return st;
+
+    if ((st = hip_check(launch_twiddles(d_tw_, d_wl_, nf, nullptr), "twiddle_kernel")) != CORR_OK)
+        return st;
+    return hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize(twiddles)");
+}
+
+Engine::~Engine() {
+    hipFree(d_tw_);
+    hipFree(d_wl_);
+    hipFree(d_spec_);
+    hipFree(d_bpairs_);
+}
+
+corr_status Engine::fengine(const int8_t *d_in, size_t station_pitch, const uint8_t *d_mask,
+                            size_t mask_pitch, int mask_bit0, int frames, hipStream_t s) {
+    if (frames < 0 || frames > shape_.chunk_frames)
+        return failf(CORR_ERR_ARG, "fengine: frames=%d outside [0, %d]", frames,
+                     shape_.chunk_frames);
+    if (d_in == nullptr) return fail(CORR_ERR_ARG, "fengine: input is NULL");
+    if (station_pitch < (size_t)frames * 2u * (size_t)shape_.n_freq)
+        return fail(CORR_ERR_ARG, "fengine: station pitch shorter than the frames it holds");
+    if (mask_bit0 < 0) return fail(CORR_ERR_ARG, "fengine: negative mask bit offset");
+    if (d_mask != nullptr && mask_pitch * 8u < (size_t)(mask_bit0 + frames))
+        return fail(CORR_ERR_ARG, "fengine: mask pitch shorter than the bits it must hold");
+    return hip_check(launch_fengine_fp32(log2_nf_, d_in, station_pitch, d_mask, mask_pitch,
+                                         mask_bit0, d_tw_, d_wl_,
+                                         static_cast<float2 *>(d_spec_),
+                                         spectra_station_stride(), frames, shape_.n_station, s),
+                     "fengine_fp32_kernel");
+}
+
+corr_status Engine::xengine_fp32(float2 *acc, int frames, hipStream_t s) {
+    if (acc == nullptr) return fail(CORR_ERR_ARG, "xengine: accumulator is NULL");
+    if (frames < 0 || frames > shape_.chunk_frames)
+        return failf(CORR_ERR_ARG, "xengine: frames=%d outside [0, %d]", frames,
+                     shape_.chunk_frames);
+    return hip_check(launch_xengine_fp32(static_cast<const float2 *>(d_spec_),
+                                         spectra_station_stride(), shape_.n_freq, frames,
+                                         shape_.n_station, d_bpairs_, n_bpairs_, x_npair_,
+                                         x_waves_, x_groups_, x_frames_lds_, acc, s),
+                     "xengine_fp32_kernel");
+}
+
+corr_status Engine::scale(float2 *acc, size_t n, uint64_t frames, hipStream_t s) {
+    if (frames == 0) return fail(CORR_ERR_ARG, "scale: frames must be > 0");
+    // 1.0f/(float)frames, then a multiply, in FP32 (reference kernels.cu:890-891).
+    const float inv = 1.0f / (float)frames;
+    return hip_check(launch_scale(acc, n, inv, s), "scale_kernel");
+}
+
+}  // namespace corr

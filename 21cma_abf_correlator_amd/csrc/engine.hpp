@@ -1,0 +1,117 @@
+// engine.hpp -- the device engine behind the drop-in ABI and the engine entry points.
+//
+// One Engine lives on one device and owns what every stage needs there: the FFT
+// twiddle tables, the baseline tables, the spectrum buffer of one chunk and, for
+// the precision forms, the requantization gains. The stages are plain enqueue
+// calls on a caller-given stream; nothing here synchronises the host.
+//
+// Stages (reference kernels.cu:810-845 does the same work as zero + convert +
+// cuFFT + corr_kernel; here it is two hand-written kernels):
+//   fengine   : validity mask + int8 -> f32 + R2C FFT (length 2*n_freq, Nyquist
+//               dropped) + the form's output conversion, in one kernel
+//   xengine   : acc[b][c] += sum over the chunk's frames of Y_j conj(Y_k)
+//   scale     : acc *= 1.0f/frames (reference kernels.cu:178-185, 890-891)
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "corr_cuda.h"
+
+namespace corr {
+
+std::string &last_error();
+corr_status fail(corr_status st, const std::string &msg);
+corr_status failf(corr_status st, const char *fmt, ...);
+corr_status hip_check(hipError_t e, const char *what);
+
+/// Spectrum representation between the F- and X-engine.
+enum class Form : int {
+    kFP32 = 0,  ///< complex64 spectra, FP32 vector X-engine (the drop-in numerics)
+    kFP16 = 1,  ///< FP16 complex spectra with per-(input,channel) gains, FP16 MFMA
+    kINT8 = 2,  ///< int8 complex spectra clipped to [-7,7] with gains, int8 MFMA
+};
+
+/// Largest station count the drop-in ABI accepts on one device (1024 baselines,
+/// reference kernels.cu:108-110, 474-483).
+constexpr int kMaxBaselinesAbi = 1024;
+
+/// FFT sizes this build transforms: n_freq = 2^k with 512 <= n_freq <= 16384.
+bool fft_size_supported(int n_freq);
+
+struct EngineShape {
+    int n_station = 0;
+    int n_freq = 0;
+    int chunk_frames = 0;   ///< capacity of the spectrum buffer, in frames
+    Form form = Form::kFP32;
+};
+
+class Engine {
+  public:
+    /// Create on the current device. Fails with CORR_ERR_NOT_IMPL for an FFT
+    /// length this build does not transform.
+    static corr_status create(const EngineShape &shape, Engine **out);
+    ~Engine();
+
+    const EngineShape &shape() const { return shape_; }
+    int n_baseline() const { return n_baseline_; }
+
+    /// F-engine over `frames` frames of every station.
+    ///   d_in   : frame f of station s at d_in + s*station_pitch + f*2*n_freq
+    ///   d_mask : optional; bit (mask_bit0 + f) of d_mask + s*mask_pitch, LSB
+    ///            first within a byte, set = present; NULL = all present
+    corr_status fengine(const int8_t *d_in, size_t station_pitch, const uint8_t *d_mask,
+                        size_t mask_pitch, int mask_bit0, int frames, hipStream_t s);
+
+    /// acc[b*n_freq + c] (complex64) += sum over the last fengine's frames.
+    corr_status xengine_fp32(float2 *acc, int frames, hipStream_t s);
+
+    /// acc *= 1.0f/(float)frames over `n` complex values.
+    static corr_status scale(float2 *acc, size_t n, uint64_t frames, hipStream_t s);
+
+    /// Spectrum buffer: [station][chunk_frames][n_freq] of the form's element.
+    void *spectra() const { return d_spec_; }
+    size_t spectra_station_stride() const { return (size_t)shape_.chunk_frames * shape_.n_freq; }
+
+    /// Device-side launch statistics for the last X-engine plan (for reports).
+    int xengine_waves() const { return x_waves_; }
+
+  private:
+    Engine() = default;
+    corr_status init();
+
+    EngineShape shape_{};
+    int n_baseline_ = 0;
+    int log2_nf_ = 0;
+
+    float2 *d_tw_ = nullptr;    ///< exp(-2 pi i m / n_freq), m < n_freq
+    float2 *d_wl_ = nullptr;    ///< exp(-2 pi i k / (2 n_freq)), k < n_freq
+    void *d_spec_ = nullptr;
+
+    // FP32 X-engine plan: 4x4 station blocks, lower-triangle block pairs.
+    int2 *d_bpairs_ = nullptr;
+    int n_bpairs_ = 0;
+    int x_npair_ = 1;   ///< block pairs per thread
+    int x_waves_ = 1;   ///< waves per workgroup
+    int x_groups_ = 1;  ///< workgroups along the pair axis
+    int x_frames_lds_ = 4;
+};
+
+// Kernel launchers (engine_kernels.hip).
+hipError_t launch_twiddles(float2 *tw, float2 *wl, int n_freq, hipStream_t s);
+hipError_t launch_fengine_fp32(int log2_nf, const int8_t *in, size_t station_pitch,
+                               const uint8_t *mask, size_t mask_pitch, int mask_bit0,
+                               const float2 *tw, const float2 *wl, float2 *out,
+                               size_t out_station_stride, int frames, int n_station,
+                               hipStream_t s);
+hipError_t launch_xengine_fp32(const float2 *y, size_t y_station_stride, int n_freq,
+                               int frames, int n_station, const int2 *bpairs, int n_bpairs,
+                               int npair, int waves, int groups, int frames_lds, float2 *acc,
+                               hipStream_t s);
+hipError_t launch_scale(float2 *acc, size_t n, float s, hipStream_t st);
+
+}  // namespace corr

@@ -1,0 +1,71 @@
+/*
+ * corrx.h -- engine-level entry points of the MI355X correlator library.
+ *
+ * The drop-in ABI (corr_cuda.h) takes frames through a pinned staging ring and
+ * returns rows through the host. These calls expose the same device stages on
+ * device-resident buffers, launched on a caller-supplied HIP stream, for
+ * callers that already hold their samples in HBM: the benchmark, the parity
+ * tests, and geometries the drop-in ABI refuses by its packet rule
+ * (n_freq % 2048 != 0, e.g. n_freq = 1024). They have no counterpart in the
+ * reference's ABI; the work they run is the reference's per-chunk device work
+ * (corr-cuda/src/kernels.cu:810-845) and row scale (kernels.cu:878-903).
+ *
+ * Plain pointers and sizes only. `stream` is a hipStream_t passed as void*
+ * (NULL = the null stream). Device pointers must belong to the device that was
+ * current when the engine was created; that device must be current when a stage
+ * is called.
+ */
+#ifndef CORRX_H
+#define CORRX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "corr_cuda.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Spectrum representation between the F- and the X-engine. */
+#define CORRX_FORM_FP32 0 /* complex64 spectra, FP32 X-engine: the drop-in numerics */
+#define CORRX_FORM_FP16 1 /* FP16 spectra, per-(input,channel) gains, FP16 MFMA     */
+#define CORRX_FORM_INT8 2 /* int8 spectra in [-7,7], gains, int8 MFMA, exact int32 */
+
+typedef struct corrx_shape {
+    int32_t n_station;    /* inputs, >= 1                                    */
+    int32_t n_freq;       /* retained channels; power of two in [512, 16384] */
+    int32_t chunk_frames; /* most frames one fengine/xengine call handles    */
+    int32_t form;         /* CORRX_FORM_*                                    */
+} corrx_shape;
+
+/* Create an engine on the current device: FP64-built twiddle tables, baseline
+   tables, a spectrum buffer of chunk_frames frames. */
+corr_status corrx_engine_create(const corrx_shape *shape, void **out_engine);
+corr_status corrx_engine_destroy(void *engine);
+
+/* F-engine: mask + int8 -> f32 + R2C FFT of `frames` frames of every station.
+   Frame f of station s is at d_in + s*station_pitch + f*2*n_freq (bytes).
+   d_mask may be NULL (every frame present); otherwise frame f of station s is
+   present when bit (mask_bit0 + f) of d_mask + s*mask_pitch is set, bit b
+   being (byte b>>3 >> (b&7)) & 1. A missing frame is transformed as zeros. */
+corr_status corrx_fengine(void *engine, const int8_t *d_in, size_t station_pitch,
+                          const uint8_t *d_mask, size_t mask_pitch, int32_t mask_bit0,
+                          int32_t frames, void *stream);
+
+/* X-engine: d_acc[b*n_freq + c] (complex64) += sum over the frames of the last
+   fengine call of Y_j conj(Y_k), baseline b = (j,k) in corr_baseline_pairs order. */
+corr_status corrx_xengine(void *engine, void *d_acc, int32_t frames, void *stream);
+
+/* d_acc *= 1.0f/(float)frames over n_complex values. */
+corr_status corrx_scale(void *d_acc, size_t n_complex, uint64_t frames, void *stream);
+
+/* The engine's spectrum buffer: [station][chunk_frames][n_freq] elements of the
+   form (complex64 for CORRX_FORM_FP32). */
+corr_status corrx_spectra(void *engine, void **d_spectra, size_t *station_stride_elems);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CORRX_H */
