@@ -4,7 +4,7 @@
 over the same 20 calls its FFI binds (corr-rs/src/ffi.rs:157-219), plus the
 asynchronous halves the C tests use (corr-cuda/include/corr_cuda.h:399-520).
 `Engine` binds the engine-level entry points of include/corrx.h, which take
-device pointers (integers, e.g. torch's data_ptr()) and a HIP stream handle.
+device pointers (integers, e.g. hip.Buffer.ptr) and a HIP stream handle.
 
 There is no CPU fallback: if the library is missing this module raises.
 """
@@ -94,14 +94,36 @@ ENGINE_SYMBOLS = (
 _lib = None
 
 
+def _mapped(pattern):
+    """Distinct files whose path contains `pattern` mapped into this process."""
+    try:
+        with open("/proc/self/maps") as f:
+            return sorted({ln.split()[-1] for ln in f
+                           if pattern in ln and ln.split()[-1].startswith("/")})
+    except OSError:
+        return []
+
+
 def load():
-    """Load libcorr_cuda.so and declare every signature. Raises if it is not built."""
+    """Load libcorr_cuda.so and declare every signature. Raises if it is not built,
+    or if the process already runs another HIP runtime (e.g. one bundled with a
+    Python wheel): the library's code objects are built for the runtime it links,
+    and two runtimes in one process do not share devices, memory or streams."""
     global _lib
     if _lib is not None:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    before = _mapped("libamdhip64")
+    if before and not _mapped(os.path.basename(LIB_PATH)):
+        raise RuntimeError(
+            f"another HIP runtime is already loaded in this process ({before[0]}); load "
+            f"libcorr_cuda before any library that brings its own HIP runtime, and do not "
+            f"share device memory or streams with it")
     L = ctypes.CDLL(LIB_PATH)
+    after = _mapped("libamdhip64")
+    if len(after) != 1:
+        raise RuntimeError(f"expected one HIP runtime after loading libcorr_cuda, found {after}")
     vp, i32, i64, u64, sz = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64, \
         ctypes.c_size_t
     st = ctypes.c_int

@@ -1,6 +1,9 @@
 // engine.cpp -- host side of the device engine: tables, plans, stage launches.
 #include "engine.hpp"
 
+#define CORR_HD __host__ __device__
+#include "xengine_body.hpp"
+
 #include <cstdarg>
 #include <cstdio>
 #include <new>
@@ -84,24 +87,17 @@ corr_status Engine::init() {
     if ((st = hip_check(hipMalloc(&d_spec_, spec_bytes), "hipMalloc(spectra)")) != CORR_OK)
         return st;
 
-    // X-engine plan: stations in blocks of 4, block pairs (jb <= kb) row major.
-    const int nb = (N + 3) / 4;
-    std::vector<int2> bp;
-    for (int jb = 0; jb < nb; ++jb)
-        for (int kb = jb; kb < nb; ++kb) bp.push_back(make_int2(jb, kb));
-    n_bpairs_ = (int)bp.size();
-    x_npair_ = n_bpairs_ > 16 ? 2 : 1;
-    const int slots = (n_bpairs_ + x_npair_ - 1) / x_npair_;
-    x_waves_ = slots < 16 ? slots : 16;
-    x_groups_ = (slots + x_waves_ - 1) / x_waves_;
-    const int npad = nb * 4;
-    x_frames_lds_ = 49152 / (npad * 64 * (int)sizeof(float2));
-    if (x_frames_lds_ < 1) x_frames_lds_ = 1;
-    if (x_frames_lds_ > 8) x_frames_lds_ = 8;
-    if ((st = hip_check(hipMalloc(reinterpret_cast<void **>(&d_bpairs_), sizeof(int2) * bp.size()),
+    // X-engine plan: stations in blocks of 4, lower-triangle block pairs.
+    const xeng::Plan plan = xeng::make_plan(N);
+    n_bpairs_ = (int)plan.bpairs.size();
+    x_npair_ = plan.npair;
+    x_waves_ = plan.waves;
+    x_groups_ = plan.groups;
+    x_frames_lds_ = plan.frames_lds;
+    if ((st = hip_check(hipMalloc(reinterpret_cast<void **>(&d_bpairs_), sizeof(int2) * plan.bpairs.size()),
                         "hipMalloc(block pairs)")) != CORR_OK)
         return st;
-    if ((st = hip_check(hipMemcpy(d_bpairs_, bp.data(), sizeof(int2) * bp.size(),
+    if ((st = hip_check(hipMemcpy(d_bpairs_, plan.bpairs.data(), sizeof(int2) * plan.bpairs.size(),
                                   hipMemcpyHostToDevice),
                         "hipMemcpy(block pairs)")) != CORR_OK)
         return st;

@@ -1,0 +1,238 @@
+// fft_body.hpp -- the F-engine's per-workgroup algorithm, written once for the
+// gfx950 kernel and for a CPU build of the same code (one host thread per lane,
+// a barrier object for the workgroup barrier), so that every index of the
+// kernel can be exercised with bounds checking on the CPU.
+//
+// The includer provides float2/char2/make_float2 (HIP's vector types on the
+// device) and defines CORR_HD (__host__ __device__ under hipcc, empty on the CPU).
+//
+// Algorithm (the reference's cuFFT R2C, corr-cuda/src/kernels.cu:578-592, 832-835,
+// restated from the mathematics): the 2*N real samples of a frame are read as N
+// complex values z[m] = x[2m] + i x[2m+1]; an N-point complex FFT is done as a
+// Stockham autosort FFT (radix-16 passes, one last radix-2/4/8 pass), each pass a
+// register-resident radix-R DFT between LDS exchanges; then the N retained R2C bins
+//   X[k] = (Z[k] + conj(Z[N-k]))/2 + W^k (Z[k] - conj(Z[N-k]))/(2i), W = exp(-i pi/N)
+// are formed (the Nyquist bin k = N is never computed).
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+
+namespace corr {
+namespace fft {
+
+CORR_HD inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+CORR_HD inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+CORR_HD inline float2 cmul(float2 a, float2 w) {
+    return make_float2(a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x);
+}
+
+// cos(2 pi m / 32) for m = 0..8, correctly rounded to float.
+CORR_HD constexpr float cos_oct(int m) {
+    return m == 0   ? 1.0f
+           : m == 1 ? 0.98078528040323044913f
+           : m == 2 ? 0.92387953251128675613f
+           : m == 3 ? 0.83146961230254523708f
+           : m == 4 ? 0.70710678118654752440f
+           : m == 5 ? 0.55557023301960222474f
+           : m == 6 ? 0.38268343236508977173f
+           : m == 7 ? 0.19509032201612826785f
+                    : 0.0f;
+}
+
+CORR_HD constexpr float cos32(int m) {
+    return m <= 8 ? cos_oct(m) : m <= 16 ? -cos_oct(16 - m) : m <= 24 ? -cos_oct(m - 16) : cos_oct(32 - m);
+}
+
+// x * exp(-2 pi i M / 32) for a compile-time M in [0, 16).
+template <int M>
+CORR_HD inline float2 rot32(float2 x) {
+    if constexpr (M == 0) {
+        return x;
+    } else if constexpr (M == 8) {  // -i
+        return make_float2(x.y, -x.x);
+    } else if constexpr (M == 4) {  // (1 - i)/sqrt2
+        constexpr float r = cos_oct(4);
+        return make_float2((x.x + x.y) * r, (x.y - x.x) * r);
+    } else if constexpr (M == 12) {  // (-1 - i)/sqrt2
+        constexpr float r = cos_oct(4);
+        return make_float2((x.y - x.x) * r, -(x.x + x.y) * r);
+    } else {
+        constexpr float c = cos32(M);
+        constexpr float s = cos32((M + 24) & 31);  // sin(2 pi M / 32)
+        return make_float2(x.x * c + x.y * s, x.y * c - x.x * s);
+    }
+}
+
+// One radix-2 DIF stage of an in-register DFT of size R: blocks of 2*H.
+template <int R, int H, int BASE, int K>
+CORR_HD inline void dif_butterflies(float2 *v) {
+    if constexpr (BASE < R) {
+        if constexpr (K < H) {
+            const float2 a = v[BASE + K], b = v[BASE + K + H];
+            v[BASE + K] = cadd(a, b);
+            v[BASE + K + H] = rot32<K * (32 / (2 * H))>(csub(a, b));  // W_{2H}^K
+            dif_butterflies<R, H, BASE, K + 1>(v);
+        } else {
+            dif_butterflies<R, H, BASE + 2 * H, 0>(v);
+        }
+    }
+}
+
+template <int R, int H>
+CORR_HD inline void dif_stages(float2 *v) {
+    if constexpr (H >= 1) {
+        dif_butterflies<R, H, 0, 0>(v);
+        dif_stages<R, H / 2>(v);
+    }
+}
+
+template <int R>
+CORR_HD constexpr int bitrev(int i) {
+    int r = 0;
+    for (int b = 1; b < R; b <<= 1) {
+        r = (r << 1) | (i & 1);
+        i >>= 1;
+    }
+    return r;
+}
+
+template <int R, int I>
+CORR_HD inline void unscramble(float2 *v) {
+    if constexpr (I < R) {
+        constexpr int J = bitrev<R>(I);
+        if constexpr (J > I) {
+            const float2 t = v[I];
+            v[I] = v[J];
+            v[J] = t;
+        }
+        unscramble<R, I + 1>(v);
+    }
+}
+
+// In-place DFT of R (power of two, <= 32) values, natural order in and out:
+// v[k] := sum_r v[r] exp(-2 pi i r k / R).
+template <int R>
+CORR_HD inline void dft(float2 *v) {
+    dif_stages<R, R / 2>(v);
+    unscramble<R, 0>(v);
+}
+
+// LDS index padding: one spare float2 every 16, so that a wave writing with a
+// stride of 16 elements (the first Stockham pass) spreads over all banks.
+CORR_HD inline int pad16(int a) { return a + (a >> 4); }
+
+template <int N>
+struct Sizes {
+    static constexpr int kThreads = N / 16;
+    static constexpr int kLds = N + N / 16;  // float2 elements
+};
+
+// One Stockham pass of radix R over N points, NS = points already combined,
+// in place in `lds` (barrier between all reads and any write). Thread j does
+// 16/R butterflies jj = j + q*N/16.
+template <int N, int R, int NS, class Sync>
+CORR_HD inline void stockham_pass(float2 (&v)[16], float2 *lds, const float2 *tw, int j, Sync &sync) {
+    constexpr int NT = N / 16;
+    constexpr int NB = 16 / R;
+    constexpr int STRIDE = N / R;
+    constexpr int TWSTEP = N / (NS * R);
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+        const int jj = j + q * NT;
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[q * R + r] = lds[pad16(jj + r * STRIDE)];
+    }
+    sync();
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+        const int jj = j + q * NT;
+        const int k = jj & (NS - 1);
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[q * R + r] = cmul(v[q * R + r], tw[r * k * TWSTEP]);
+        dft<R>(&v[q * R]);
+        const int base = (jj - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) lds[pad16(base + r * NS)] = v[q * R + r];
+    }
+    sync();
+}
+
+template <int LOG2N, int DONE, class Sync>
+CORR_HD inline void stockham_rest(float2 (&v)[16], float2 *lds, const float2 *tw, int j, Sync &sync) {
+    if constexpr (DONE < LOG2N) {
+        constexpr int REM = LOG2N - DONE;
+        constexpr int LOGR = REM >= 4 ? 4 : REM;
+        stockham_pass<(1 << LOG2N), (1 << LOGR), (1 << DONE)>(v, lds, tw, j, sync);
+        stockham_rest<LOG2N, DONE + LOGR>(v, lds, tw, j, sync);
+    }
+}
+
+// Validity of frame f of station st: bit (bit0 + f) of mask + st*mask_pitch.
+CORR_HD inline bool frame_present(const uint8_t *mask, size_t mask_pitch, int bit0, int f, int st) {
+    if (mask == nullptr) return true;
+    const int bit = bit0 + f;
+    return ((mask[(size_t)st * mask_pitch + (size_t)(bit >> 3)] >> (bit & 7)) & 1u) != 0;
+}
+
+// The Z spectrum of frame f of station st into `lds` (natural order, padded).
+// Thread j of N/16; on return `lds` holds Z after a final barrier.
+template <int LOG2N, class Sync>
+CORR_HD inline void complex_fft_block(int j, int f, int st, float2 *lds, Sync &sync, const int8_t *in,
+                                      size_t station_pitch, const uint8_t *mask, size_t mask_pitch,
+                                      int mask_bit0, const float2 *tw) {
+    constexpr int N = 1 << LOG2N;
+    constexpr int NT = N / 16;
+    float2 v[16];
+    if (frame_present(mask, mask_pitch, mask_bit0, f, st)) {
+        const char2 *src = reinterpret_cast<const char2 *>(in + (size_t)st * station_pitch +
+                                                           (size_t)f * (size_t)(2 * N));
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const char2 p = src[j + r * NT];
+            v[r] = make_float2((float)p.x, (float)p.y);
+        }
+    } else {
+        // A missing frame is never read: zeros are transformed instead, which is
+        // bit for bit what a frame the host had zeroed gives.
+#pragma unroll
+        for (int r = 0; r < 16; ++r) v[r] = make_float2(0.0f, 0.0f);
+    }
+    // first pass: radix 16, NS = 1, inputs z[j + r*N/16] already in registers
+    dft<16>(v);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) lds[pad16(j * 16 + r)] = v[r];
+    sync();
+    stockham_rest<LOG2N, 4>(v, lds, tw, j, sync);
+}
+
+// R2C bin k from Z in lds: (Z[k] + conj(Z[N-k]))/2 + W^k (Z[k] - conj(Z[N-k]))/(2i).
+template <int N>
+CORR_HD inline float2 r2c_bin(const float2 *lds, const float2 *wl, int k) {
+    const float2 a = lds[pad16(k)];
+    const float2 b = lds[pad16((N - k) & (N - 1))];
+    const float er = 0.5f * (a.x + b.x), ei = 0.5f * (a.y - b.y);
+    const float orr = 0.5f * (a.y + b.y), oi = -0.5f * (a.x - b.x);
+    const float2 w = wl[k];
+    return make_float2(er + (w.x * orr - w.y * oi), ei + (w.x * oi + w.y * orr));
+}
+
+// Whole F-engine workgroup, complex64 output [station][frame][N].
+template <int LOG2N, class Sync>
+CORR_HD inline void fengine_fp32_block(int j, int f, int st, float2 *lds, Sync &sync, const int8_t *in,
+                                       size_t station_pitch, const uint8_t *mask, size_t mask_pitch,
+                                       int mask_bit0, const float2 *tw, const float2 *wl, float2 *out,
+                                       size_t out_station_stride) {
+    constexpr int N = 1 << LOG2N;
+    constexpr int NT = N / 16;
+    complex_fft_block<LOG2N>(j, f, st, lds, sync, in, station_pitch, mask, mask_pitch, mask_bit0, tw);
+    float2 *dst = out + (size_t)st * out_station_stride + (size_t)f * (size_t)N;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int k = j + q * NT;
+        dst[k] = r2c_bin<N>(lds, wl, k);
+    }
+}
+
+}  // namespace fft
+}  // namespace corr
