@@ -124,6 +124,12 @@ def load():
     after = _mapped("libamdhip64")
     if len(after) != 1:
         raise RuntimeError(f"expected one HIP runtime after loading libcorr_cuda, found {after}")
+    _lib = declare(L)
+    return _lib
+
+
+def declare(L):
+    """Declare the argument and result types of every exported call on `L`."""
     vp, i32, i64, u64, sz = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64, \
         ctypes.c_size_t
     st = ctypes.c_int
@@ -165,20 +171,19 @@ def load():
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = res
-    _lib = L
     return L
 
 
 class CorrError(RuntimeError):
-    def __init__(self, status, what):
+    def __init__(self, status, what, lib=None):
         self.status = status
-        msg = load().corr_last_error().decode(errors="replace")
+        msg = (lib or load()).corr_last_error().decode(errors="replace")
         super().__init__(f"{what}: {STATUS_NAMES.get(status, status)}: {msg}")
 
 
-def check(status, what):
+def check(status, what, lib=None):
     if status != OK:
-        raise CorrError(status, what)
+        raise CorrError(status, what, lib)
 
 
 def abi_banner():
@@ -223,29 +228,33 @@ def make_config(n_station, n_freq, frames_per_slot, chunk_frames, block_slots=3,
     return c
 
 
-def create_raw(cfg):
+def create_raw(cfg, lib=None):
     """corr_create; returns (status, handle or None) without raising."""
     h = ctypes.c_void_p()
-    s = load().corr_create(ctypes.byref(cfg), ctypes.byref(h))
+    s = (lib or load()).corr_create(ctypes.byref(cfg), ctypes.byref(h))
     return s, (h.value if h.value else None)
 
 
 class Correlator:
-    """One correlator context (reference corr-rs/src/gpu.rs:32-333)."""
+    """One correlator context (reference corr-rs/src/gpu.rs:32-333). `lib` is the
+    loaded library to drive (default: this package's libcorr_cuda.so)."""
 
-    def __init__(self, cfg):
+    def __init__(self, cfg, lib=None):
         self.cfg = cfg
-        s, h = create_raw(cfg)
-        check(s, "corr_create")
+        self._lib = lib or load()
+        s, h = create_raw(cfg, self._lib)
+        check(s, "corr_create", self._lib)
         self.h = h
-        self._lib = load()
         self._staging = [self.staging(i) for i in range(cfg.n_station)]
+
+    def _check(self, status, what):
+        check(status, what, self._lib)
 
     def close(self):
         if self.h is not None:
             s = self._lib.corr_destroy(self.h)
             self.h = None
-            check(s, "corr_destroy")
+            self._check(s, "corr_destroy")
 
     def __enter__(self):
         return self
@@ -280,7 +289,7 @@ class Correlator:
 
     def staging(self, station):
         out = corr_staging()
-        check(self._lib.corr_staging_info(self.h, station, ctypes.byref(out)), "corr_staging_info")
+        self._check(self._lib.corr_staging_info(self.h, station, ctypes.byref(out)), "corr_staging_info")
         return out
 
     def station_view(self, station):
@@ -303,20 +312,20 @@ class Correlator:
 
     # --- row lifecycle ------------------------------------------------------
     def row_accumulate(self, slot_seq, frames):
-        check(self._lib.corr_row_accumulate(self.h, slot_seq, frames), "corr_row_accumulate")
+        self._check(self._lib.corr_row_accumulate(self.h, slot_seq, frames), "corr_row_accumulate")
 
     def row_slot_free(self, slot_seq):
-        check(self._lib.corr_row_slot_free(self.h, slot_seq), "corr_row_slot_free")
+        self._check(self._lib.corr_row_slot_free(self.h, slot_seq), "corr_row_slot_free")
 
     def row_end(self, frames):
-        check(self._lib.corr_row_end(self.h, frames), "corr_row_end")
+        self._check(self._lib.corr_row_end(self.h, frames), "corr_row_end")
 
     def rows_wait(self):
-        check(self._lib.corr_rows_wait(self.h), "corr_rows_wait")
+        self._check(self._lib.corr_rows_wait(self.h), "corr_rows_wait")
 
     def rows_pending(self):
         p = ctypes.c_int32(-1)
-        check(self._lib.corr_rows_pending(self.h, ctypes.byref(p)), "corr_rows_pending")
+        self._check(self._lib.corr_rows_pending(self.h, ctypes.byref(p)), "corr_rows_pending")
         return p.value
 
     def rows_fetch(self, out=None):
@@ -324,13 +333,13 @@ class Correlator:
         if out is None:
             out = np.empty(self.fetch_bytes() // 8, dtype=np.complex64)
         rows = ctypes.c_int32(-1)
-        check(self._lib.corr_rows_fetch(self.h, out.ctypes.data, out.nbytes, ctypes.byref(rows)),
+        self._check(self._lib.corr_rows_fetch(self.h, out.ctypes.data, out.nbytes, ctypes.byref(rows)),
               "corr_rows_fetch")
         return rows.value, out
 
     def device_stats(self, device_index=0):
         s = corr_device_stats()
-        check(self._lib.corr_device_stats_get(self.h, device_index, ctypes.byref(s)),
+        self._check(self._lib.corr_device_stats_get(self.h, device_index, ctypes.byref(s)),
               "corr_device_stats_get")
         return s
 
@@ -339,7 +348,7 @@ class Correlator:
         plans = []
         for d in range(n):
             p = corr_device_plan()
-            check(self._lib.corr_device_plan_get(self.h, d, ctypes.byref(p)), "corr_device_plan_get")
+            self._check(self._lib.corr_device_plan_get(self.h, d, ctypes.byref(p)), "corr_device_plan_get")
             plans.append(p)
         return plans
 

@@ -1,7 +1,9 @@
 // engine.cpp -- host side of the device engine: tables, plans, stage launches.
 #include "engine.hpp"
 
+#ifndef CORR_HD
 #define CORR_HD __host__ __device__
+#endif
 #include "xengine_body.hpp"
 
 #include <cstdarg>

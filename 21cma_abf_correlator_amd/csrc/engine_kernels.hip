@@ -23,7 +23,9 @@
 //   complex accumulators per block and does 64 FMAs per 8 LDS reads.
 #include "engine.hpp"
 
+#ifndef CORR_HD
 #define CORR_HD __host__ __device__
+#endif
 #include "fft_body.hpp"
 #include "xengine_body.hpp"
 
@@ -141,8 +143,7 @@ hipError_t launch_xengine_fp32(const float2 *y, size_t y_station_stride, int n_f
                                int waves, int groups, int frames_lds, float2 *acc,
                                hipStream_t s) {
     if (frames <= 0) return hipSuccess;
-    const int npad = (n_station + 3) & ~3;
-    const size_t shm = (size_t)npad * (size_t)frames_lds * 64 * sizeof(float2);
+    const size_t shm = xeng::lds_bytes(n_station, frames_lds);
     const dim3 grid((unsigned)((n_freq + 63) / 64), (unsigned)groups);
     const dim3 block((unsigned)(64 * waves));
     switch (npair) {

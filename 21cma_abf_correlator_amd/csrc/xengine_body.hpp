@@ -30,6 +30,12 @@ struct Plan {
     size_t lds_bytes = 0;
 };
 
+/// Dynamic LDS bytes of one workgroup: [npad][frames_lds][64] complex64.
+CORR_HD inline size_t lds_bytes(int n_station, int frames_lds) {
+    const int npad = (n_station + 3) & ~3;
+    return (size_t)npad * (size_t)frames_lds * 64 * 8;
+}
+
 inline Plan make_plan(int n_station) {
     Plan p;
     const int nb = (n_station + 3) / 4;
@@ -44,7 +50,7 @@ inline Plan make_plan(int n_station) {
     p.frames_lds = 49152 / (npad * 64 * 8);
     if (p.frames_lds < 1) p.frames_lds = 1;
     if (p.frames_lds > 8) p.frames_lds = 8;
-    p.lds_bytes = (size_t)npad * (size_t)p.frames_lds * 64 * 8;
+    p.lds_bytes = lds_bytes(n_station, p.frames_lds);
     return p;
 }
 

@@ -13,11 +13,11 @@ def frame_is_missing(row, frame_in_row):
     return ((row * 3 + frame_in_row) % MISSING_EVERY) == MISSING_PHASE
 
 
-def drive(corr, cfg, x, rows, n_cum, mode=ALL_VALID):
+def drive(corr, cfg, x, rows, n_cum, mode=ALL_VALID, lib=None):
     """x: int8 (rows, N, n_cum, L). Returns complex64 (rows, B, nf) from one fetch."""
     fps = cfg.frames_per_slot
     N = cfg.n_station
-    with corr.Correlator(cfg) as c:
+    with corr.Correlator(cfg, lib=lib) as c:
         submits = 0
         for r in range(rows):
             for f0 in range(0, n_cum, fps):

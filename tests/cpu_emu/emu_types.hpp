@@ -15,7 +15,9 @@ struct int2 {
 };
 inline int2 make_int2(int x, int y) { return int2{x, y}; }
 
+#ifndef CORR_HD
 #define CORR_HD
+#endif
 
 // The workgroup barrier of the emulation: every emulated lane is a thread.
 struct HostSync {
