@@ -88,6 +88,10 @@ corr_status Engine::init() {
         (size_t)N * (size_t)shape_.chunk_frames * (size_t)nf * sizeof(float2);
     if ((st = hip_check(hipMalloc(&d_spec_, spec_bytes), "hipMalloc(spectra)")) != CORR_OK)
         return st;
+    if ((st = hip_check(hipMalloc(reinterpret_cast<void **>(&d_valid_),
+                                  sizeof(int32_t) * (size_t)N * (size_t)shape_.chunk_frames),
+                        "hipMalloc(validity)")) != CORR_OK)
+        return st;
 
     // X-engine plan: stations in blocks of 4, lower-triangle block pairs.
     const xeng::Plan plan = xeng::make_plan(N);
@@ -113,6 +117,7 @@ Engine::~Engine() {
     hipFree(d_tw_);
     hipFree(d_wl_);
     hipFree(d_spec_);
+    hipFree(d_valid_);
     hipFree(d_bpairs_);
 }
 
@@ -127,9 +132,19 @@ corr_status Engine::fengine(const int8_t *d_in, size_t station_pitch, const uint
     if (mask_bit0 < 0) return fail(CORR_ERR_ARG, "fengine: negative mask bit offset");
     if (d_mask != nullptr && mask_pitch * 8u < (size_t)(mask_bit0 + frames))
         return fail(CORR_ERR_ARG, "fengine: mask pitch shorter than the bits it must hold");
-    return hip_check(launch_fengine_fp32(log2_nf_, d_in, station_pitch, d_mask, mask_pitch,
-                                         mask_bit0, d_tw_, d_wl_,
-                                         static_cast<float2 *>(d_spec_),
+    // The validity bits are expanded to one int32 per (station, frame) by a
+    // separate kernel; the F-engine reads that flag once per workgroup.
+    const int32_t *valid = nullptr;
+    const size_t valid_stride = (size_t)shape_.chunk_frames;
+    if (d_mask != nullptr) {
+        corr_status st = hip_check(launch_validity(d_mask, mask_pitch, mask_bit0, frames,
+                                                   shape_.n_station, d_valid_, valid_stride, s),
+                                   "validity_kernel");
+        if (st != CORR_OK) return st;
+        valid = d_valid_;
+    }
+    return hip_check(launch_fengine_fp32(log2_nf_, d_in, station_pitch, valid, valid_stride, d_tw_,
+                                         d_wl_, static_cast<float2 *>(d_spec_),
                                          spectra_station_stride(), frames, shape_.n_station, s),
                      "fengine_fp32_kernel");
 }

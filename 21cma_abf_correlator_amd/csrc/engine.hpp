@@ -91,6 +91,7 @@ class Engine {
     float2 *d_tw_ = nullptr;    ///< exp(-2 pi i m / n_freq), m < n_freq
     float2 *d_wl_ = nullptr;    ///< exp(-2 pi i k / (2 n_freq)), k < n_freq
     void *d_spec_ = nullptr;
+    int32_t *d_valid_ = nullptr;  ///< expanded validity, [station][chunk_frames]
 
     // FP32 X-engine plan: 4x4 station blocks, lower-triangle block pairs.
     int2 *d_bpairs_ = nullptr;
@@ -103,11 +104,12 @@ class Engine {
 
 // Kernel launchers (engine_kernels.hip).
 hipError_t launch_twiddles(float2 *tw, float2 *wl, int n_freq, hipStream_t s);
+hipError_t launch_validity(const uint8_t *mask, size_t mask_pitch, int bit0, int frames,
+                           int n_station, int32_t *valid, size_t valid_stride, hipStream_t s);
 hipError_t launch_fengine_fp32(int log2_nf, const int8_t *in, size_t station_pitch,
-                               const uint8_t *mask, size_t mask_pitch, int mask_bit0,
-                               const float2 *tw, const float2 *wl, float2 *out,
-                               size_t out_station_stride, int frames, int n_station,
-                               hipStream_t s);
+                               const int32_t *valid, size_t valid_stride, const float2 *tw,
+                               const float2 *wl, float2 *out, size_t out_station_stride,
+                               int frames, int n_station, hipStream_t s);
 hipError_t launch_xengine_fp32(const float2 *y, size_t y_station_stride, int n_freq,
                                int frames, int n_station, const int2 *bpairs, int n_bpairs,
                                int npair, int waves, int groups, int frames_lds, float2 *acc,

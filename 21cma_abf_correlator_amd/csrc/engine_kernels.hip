@@ -43,14 +43,23 @@ struct DeviceSync {
 template <int LOG2N>
 __global__ void __launch_bounds__((1 << LOG2N) / 16)
 fengine_fp32_kernel(const int8_t *__restrict__ in, size_t station_pitch,
-                    const uint8_t *__restrict__ mask, size_t mask_pitch, int mask_bit0,
+                    const int32_t *__restrict__ valid, size_t valid_stride,
                     const float2 *__restrict__ tw, const float2 *__restrict__ wl,
                     float2 *__restrict__ out, size_t out_station_stride) {
     __shared__ float2 lds[fft::Sizes<(1 << LOG2N)>::kLds];
     DeviceSync sync;
     fft::fengine_fp32_block<LOG2N>((int)threadIdx.x, (int)blockIdx.x, (int)blockIdx.y, lds, sync, in,
-                                   station_pitch, mask, mask_pitch, mask_bit0, tw, wl, out,
+                                   station_pitch, valid, valid_stride, tw, wl, out,
                                    out_station_stride);
+}
+
+// Validity bits -> int32 per (station, frame); grid (ceil(frames/256), stations).
+constexpr unsigned kValidityBlock = 256;
+__global__ void __launch_bounds__(kValidityBlock)
+validity_kernel(const uint8_t *__restrict__ mask, size_t mask_pitch, int bit0, int frames,
+                int32_t *__restrict__ valid, size_t valid_stride) {
+    fft::expand_validity((int)(blockIdx.x * kValidityBlock + threadIdx.x), (int)blockIdx.y, frames,
+                         mask, mask_pitch, bit0, valid, valid_stride);
 }
 
 // Twiddle tables in FP64: tw[m] = exp(-2 pi i m/n), wl[k] = exp(-2 pi i k/(2n)).
@@ -95,14 +104,14 @@ __global__ void __launch_bounds__(kScaleBlock) scale_kernel(float2 *acc, size_t 
 }
 
 template <int LOG2N>
-hipError_t launch_fengine_fp32_t(const int8_t *in, size_t station_pitch, const uint8_t *mask,
-                                 size_t mask_pitch, int mask_bit0, const float2 *tw,
-                                 const float2 *wl, float2 *out, size_t out_station_stride,
-                                 int frames, int n_station, hipStream_t s) {
+hipError_t launch_fengine_fp32_t(const int8_t *in, size_t station_pitch, const int32_t *valid,
+                                 size_t valid_stride, const float2 *tw, const float2 *wl,
+                                 float2 *out, size_t out_station_stride, int frames,
+                                 int n_station, hipStream_t s) {
     const dim3 grid((unsigned)frames, (unsigned)n_station);
     const dim3 block((1u << LOG2N) / 16u);
-    hipLaunchKernelGGL(fengine_fp32_kernel<LOG2N>, grid, block, 0, s, in, station_pitch, mask,
-                       mask_pitch, mask_bit0, tw, wl, out, out_station_stride);
+    hipLaunchKernelGGL(fengine_fp32_kernel<LOG2N>, grid, block, 0, s, in, station_pitch, valid,
+                       valid_stride, tw, wl, out, out_station_stride);
     return hipGetLastError();
 }
 
@@ -115,17 +124,25 @@ hipError_t launch_twiddles(float2 *tw, float2 *wl, int n_freq, hipStream_t s) {
     return hipGetLastError();
 }
 
+hipError_t launch_validity(const uint8_t *mask, size_t mask_pitch, int bit0, int frames,
+                           int n_station, int32_t *valid, size_t valid_stride, hipStream_t s) {
+    if (frames <= 0 || n_station <= 0) return hipSuccess;
+    const dim3 grid((unsigned)((frames + kValidityBlock - 1) / kValidityBlock), (unsigned)n_station);
+    hipLaunchKernelGGL(validity_kernel, grid, dim3(kValidityBlock), 0, s, mask, mask_pitch, bit0,
+                       frames, valid, valid_stride);
+    return hipGetLastError();
+}
+
 hipError_t launch_fengine_fp32(int log2_nf, const int8_t *in, size_t station_pitch,
-                               const uint8_t *mask, size_t mask_pitch, int mask_bit0,
-                               const float2 *tw, const float2 *wl, float2 *out,
-                               size_t out_station_stride, int frames, int n_station,
-                               hipStream_t s) {
+                               const int32_t *valid, size_t valid_stride, const float2 *tw,
+                               const float2 *wl, float2 *out, size_t out_station_stride,
+                               int frames, int n_station, hipStream_t s) {
     if (frames <= 0 || n_station <= 0) return hipSuccess;
     switch (log2_nf) {
-#define CORR_FENGINE_CASE(L)                                                                \
-    case L:                                                                                 \
-        return launch_fengine_fp32_t<L>(in, station_pitch, mask, mask_pitch, mask_bit0, tw, \
-                                        wl, out, out_station_stride, frames, n_station, s);
+#define CORR_FENGINE_CASE(L)                                                              \
+    case L:                                                                               \
+        return launch_fengine_fp32_t<L>(in, station_pitch, valid, valid_stride, tw, wl, out, \
+                                        out_station_stride, frames, n_station, s);
         CORR_FENGINE_CASE(9)
         CORR_FENGINE_CASE(10)
         CORR_FENGINE_CASE(11)

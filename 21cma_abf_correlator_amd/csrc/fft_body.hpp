@@ -168,23 +168,33 @@ CORR_HD inline void stockham_rest(float2 (&v)[16], float2 *lds, const float2 *tw
     }
 }
 
-// Validity of frame f of station st: bit (bit0 + f) of mask + st*mask_pitch.
-CORR_HD inline bool frame_present(const uint8_t *mask, size_t mask_pitch, int bit0, int f, int st) {
-    if (mask == nullptr) return true;
+// Validity bit of frame f of station st: bit (bit0 + f) of mask + st*mask_pitch,
+// LSB first within a byte (reference kernels.cu:163-166).
+CORR_HD inline int32_t mask_bit(const uint8_t *mask, size_t mask_pitch, int bit0, int f, int st) {
     const int bit = bit0 + f;
-    return ((mask[(size_t)st * mask_pitch + (size_t)(bit >> 3)] >> (bit & 7)) & 1u) != 0;
+    return (int32_t)((mask[(size_t)st * mask_pitch + (size_t)(bit >> 3)] >> (bit & 7)) & 1u);
+}
+
+// Expand the validity bits of `frames` frames into one int32 per (station,
+// frame): valid[st*valid_stride + f]. Thread f of the workgroup of station st.
+CORR_HD inline void expand_validity(int f, int st, int frames, const uint8_t *mask, size_t mask_pitch,
+                                    int bit0, int32_t *valid, size_t valid_stride) {
+    if (f < frames) valid[(size_t)st * valid_stride + (size_t)f] = mask_bit(mask, mask_pitch, bit0, f, st);
 }
 
 // The Z spectrum of frame f of station st into `lds` (natural order, padded).
 // Thread j of N/16; on return `lds` holds Z after a final barrier.
+// `valid` is NULL (every frame present) or the expanded validity of
+// expand_validity(): one aligned int32 per (station, frame), read once per
+// workgroup at a uniform address.
 template <int LOG2N, class Sync>
 CORR_HD inline void complex_fft_block(int j, int f, int st, float2 *lds, Sync &sync, const int8_t *in,
-                                      size_t station_pitch, const uint8_t *mask, size_t mask_pitch,
-                                      int mask_bit0, const float2 *tw) {
+                                      size_t station_pitch, const int32_t *valid, size_t valid_stride,
+                                      const float2 *tw) {
     constexpr int N = 1 << LOG2N;
     constexpr int NT = N / 16;
     float2 v[16];
-    if (frame_present(mask, mask_pitch, mask_bit0, f, st)) {
+    if (valid == nullptr || valid[(size_t)st * valid_stride + (size_t)f] != 0) {
         const char2 *src = reinterpret_cast<const char2 *>(in + (size_t)st * station_pitch +
                                                            (size_t)f * (size_t)(2 * N));
 #pragma unroll
@@ -220,12 +230,12 @@ CORR_HD inline float2 r2c_bin(const float2 *lds, const float2 *wl, int k) {
 // Whole F-engine workgroup, complex64 output [station][frame][N].
 template <int LOG2N, class Sync>
 CORR_HD inline void fengine_fp32_block(int j, int f, int st, float2 *lds, Sync &sync, const int8_t *in,
-                                       size_t station_pitch, const uint8_t *mask, size_t mask_pitch,
-                                       int mask_bit0, const float2 *tw, const float2 *wl, float2 *out,
+                                       size_t station_pitch, const int32_t *valid, size_t valid_stride,
+                                       const float2 *tw, const float2 *wl, float2 *out,
                                        size_t out_station_stride) {
     constexpr int N = 1 << LOG2N;
     constexpr int NT = N / 16;
-    complex_fft_block<LOG2N>(j, f, st, lds, sync, in, station_pitch, mask, mask_pitch, mask_bit0, tw);
+    complex_fft_block<LOG2N>(j, f, st, lds, sync, in, station_pitch, valid, valid_stride, tw);
     float2 *dst = out + (size_t)st * out_station_stride + (size_t)f * (size_t)N;
 #pragma unroll
     for (int q = 0; q < 16; ++q) {

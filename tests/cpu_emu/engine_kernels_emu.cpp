@@ -36,15 +36,15 @@ void run_grid(int gx, int gy, int nthreads, size_t lds_bytes, Body body) {
 }
 
 template <int LOG2N>
-hipError_t fengine_t(const int8_t *in, size_t station_pitch, const uint8_t *mask, size_t mask_pitch,
-                     int mask_bit0, const float2 *tw, const float2 *wl, float2 *out,
-                     size_t out_station_stride, int frames, int n_station) {
+hipError_t fengine_t(const int8_t *in, size_t station_pitch, const int32_t *valid, size_t valid_stride,
+                     const float2 *tw, const float2 *wl, float2 *out, size_t out_station_stride,
+                     int frames, int n_station) {
     constexpr int N = 1 << LOG2N;
     run_grid(frames, n_station, N / 16, fft::Sizes<N>::kLds * sizeof(float2),
              [=](int j, int bx, int by, char *lds, HostSync &sync) {
                  fft::fengine_fp32_block<LOG2N>(j, bx, by, reinterpret_cast<float2 *>(lds), sync, in,
-                                                station_pitch, mask, mask_pitch, mask_bit0, tw, wl,
-                                                out, out_station_stride);
+                                                station_pitch, valid, valid_stride, tw, wl, out,
+                                                out_station_stride);
              });
     return hipSuccess;
 }
@@ -61,16 +61,25 @@ hipError_t launch_twiddles(float2 *tw, float2 *wl, int n_freq, hipStream_t) {
     return hipSuccess;
 }
 
+hipError_t launch_validity(const uint8_t *mask, size_t mask_pitch, int bit0, int frames,
+                           int n_station, int32_t *valid, size_t valid_stride, hipStream_t) {
+    if (frames <= 0 || n_station <= 0) return hipSuccess;
+    const int gx = (frames + 255) / 256;  // the device launcher's grid, 256 lanes
+    for (int st = 0; st < n_station; ++st)
+        for (int lane = 0; lane < gx * 256; ++lane)
+            fft::expand_validity(lane, st, frames, mask, mask_pitch, bit0, valid, valid_stride);
+    return hipSuccess;
+}
+
 hipError_t launch_fengine_fp32(int log2_nf, const int8_t *in, size_t station_pitch,
-                               const uint8_t *mask, size_t mask_pitch, int mask_bit0,
-                               const float2 *tw, const float2 *wl, float2 *out,
-                               size_t out_station_stride, int frames, int n_station,
-                               hipStream_t) {
+                               const int32_t *valid, size_t valid_stride, const float2 *tw,
+                               const float2 *wl, float2 *out, size_t out_station_stride,
+                               int frames, int n_station, hipStream_t) {
     if (frames <= 0 || n_station <= 0) return hipSuccess;
     switch (log2_nf) {
 #define E(L)  \
     case L: \
-        return fengine_t<L>(in, station_pitch, mask, mask_pitch, mask_bit0, tw, wl, out, out_station_stride, frames, n_station);
+        return fengine_t<L>(in, station_pitch, valid, valid_stride, tw, wl, out, out_station_stride, frames, n_station);
         E(9) E(10) E(11) E(12) E(13) E(14)
 #undef E
     default:

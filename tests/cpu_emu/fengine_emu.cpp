@@ -17,8 +17,8 @@ extern "C" void orc_fft_r2c_f64(const double *x, int L, double *out);
 using namespace corr::fft;
 
 template <int LOG2N>
-static void run_block(int f, int st, const int8_t *in, size_t pitch, const uint8_t *mask,
-                      size_t mpitch, int bit0, const float2 *tw, const float2 *wl, float2 *out,
+static void run_block(int f, int st, const int8_t *in, size_t pitch, const int32_t *valid,
+                      size_t vstride, const float2 *tw, const float2 *wl, float2 *out,
                       size_t ostride) {
     constexpr int N = 1 << LOG2N;
     constexpr int NT = N / 16;
@@ -29,7 +29,7 @@ static void run_block(int f, int st, const int8_t *in, size_t pitch, const uint8
     for (int j = 0; j < NT; ++j)
         th.emplace_back([=, &bar] {
             HostSync sync{&bar};
-            fengine_fp32_block<LOG2N>(j, f, st, lds, sync, in, pitch, mask, mpitch, bit0, tw, wl, out,
+            fengine_fp32_block<LOG2N>(j, f, st, lds, sync, in, pitch, valid, vstride, tw, wl, out,
                                       ostride);
         });
     for (auto &t : th) t.join();
@@ -59,9 +59,14 @@ static int check() {
     }
     const size_t ostride = (size_t)FR * N;
     float2 *out = new float2[ostride * NST];
+    // validity expanded exactly as the validity kernel does (one lane per frame)
+    int32_t *valid = new int32_t[(size_t)NST * FR];
+    for (int st = 0; st < NST; ++st)
+        for (int f = 0; f < FR + 5; ++f)  // lanes past `frames` must write nothing
+            expand_validity(f, st, FR, mask, mpitch, BIT0, valid, FR);
     for (int st = 0; st < NST; ++st)
         for (int f = 0; f < FR; ++f)
-            run_block<LOG2N>(f, st, in, pitch, mask, mpitch, BIT0, tw, wl, out, ostride);
+            run_block<LOG2N>(f, st, in, pitch, valid, FR, tw, wl, out, ostride);
 
     int bad = 0;
     std::vector<double> x(2 * N), ref(2 * N);
@@ -89,6 +94,7 @@ static int check() {
     std::printf("N=%d: worst relative RMS %.3e, %d bad\n", N, worst, bad);
     delete[] in;
     delete[] mask;
+    delete[] valid;
     delete[] tw;
     delete[] wl;
     delete[] out;

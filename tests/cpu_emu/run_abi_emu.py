@@ -43,7 +43,7 @@ def golden(L, N, nf, n_cum, fps, rows):
 
 def main():
     L = corr.declare(ctypes.CDLL(os.path.join(HERE, "_build", "libcorr_cuda_emu.so")))
-    ok = golden(L, 4, 2048, 300, 128, 2)
+    ok = golden(L, 4, 2048, 90, 32, 2)
     ok = golden(L, 5, 4096, 37, 16, 3) and ok
     print("OK" if ok else "FAILED")
     return 0 if ok else 1

@@ -6,6 +6,7 @@ match FP64 (tests/cpu_emu/)."""
 
 import os
 import subprocess
+import sys
 
 import pytest
 
@@ -25,3 +26,54 @@ def test_kernel_body_emulation(built, prog):
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "OK" in r.stdout
+
+
+def test_generated_code_has_no_clobbered_upper_halves():
+    """The gfx950 code of every kernel, compiled with the library's own flags,
+    passes the register-hazard check of tests/isa_lint.py."""
+    import isa_lint
+    csrc = os.path.join(os.path.dirname(EMU), "..", "21cma_abf_correlator_amd", "csrc")
+    subprocess.run(["make", "-s", "-C", csrc, "asm"], check=True)
+    text = open(os.path.join(csrc, "..", "lib", "obj", "engine_kernels.s")).read()
+    kernels = isa_lint.parse_kernels(text)
+    assert any("fengine_fp32_kernel" in k for k in kernels)
+    assert any("xengine_fp32_kernel" in k for k in kernels)
+    findings = isa_lint.lint_text(text)
+    assert not findings, findings
+
+
+def test_isa_lint_catches_the_known_miscompile():
+    """A load result reused as the upper half of a zero-extended index, on one
+    path only (the pattern ROCm 7.2's allocator produced for the F-engine)."""
+    import isa_lint
+    bad = """k:
+\tv_mov_b32_e32 v1, 0
+\ts_cbranch_scc0 .LBB0_2
+\tglobal_load_ubyte v1, v1, s[6:7]
+\tv_readfirstlane_b32 s6, v1
+\ts_branch .LBB0_3
+.LBB0_2:
+\tv_mov_b32_e32 v1, 0
+.LBB0_3:
+\tv_lshlrev_b64 v[40:41], 3, v[0:1]
+\tglobal_store_dwordx2 v[40:41], v[2:3], off
+\ts_endpgm
+.Lfunc_end0:
+"""
+    assert len(isa_lint.lint_text(bad)) == 1
+    ok = bad.replace("\tv_readfirstlane_b32 s6, v1\n", "\tv_readfirstlane_b32 s6, v1\n\tv_mov_b32_e32 v1, 0\n")
+    assert isa_lint.lint_text(ok) == []
+
+
+def test_abi_path_in_cpu_build(built):
+    """The whole drop-in path -- ABI layer, engine, launch geometry, copies -- in
+    the CPU build under AddressSanitizer, driven like the reference's golden
+    test: parity with the FP64 oracle and host-zeroed == mask-marked bytes."""
+    asan = subprocess.run(["gcc", "-print-file-name=libasan.so"], capture_output=True,
+                          text=True, check=True).stdout.strip()
+    env = dict(os.environ, LD_PRELOAD=asan, ASAN_OPTIONS="detect_leaks=0")
+    r = subprocess.run([sys.executable, os.path.join(EMU, "run_abi_emu.py")], capture_output=True,
+                       text=True, timeout=900, env=env)
+    print(r.stdout[-2000:])
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    assert r.stdout.strip().endswith("OK")
