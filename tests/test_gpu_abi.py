@@ -11,11 +11,7 @@ import time
 import numpy as np
 import pytest
 
-pytestmark = [
-    pytest.mark.gpu,
-    pytest.mark.skip(reason="drop-in ABI path kept off the GPU until the engine path has run "
-                            "clean (see tests/test_gpu_golden.py)"),
-]
+pytestmark = pytest.mark.gpu
 
 
 def small_cfg(corr, **kw):

@@ -16,14 +16,7 @@ import pytest
 from abi_driver import ALL_VALID, HOST_ZEROED, MASK_MARKED, drive, valid_matrix
 from conftest import rel_rms
 
-pytestmark = [
-    pytest.mark.gpu,
-    pytest.mark.skip(reason="the drop-in ABI path faulted the GPU on its first run (golden 4 "
-                            "stations, n_freq 2048); the pitched host-to-device copies it used are "
-                            "switched off and the case stays off the GPU until the engine path "
-                            "has run clean; the path is verified in the ASan CPU build "
-                            "(tests/test_cpu_emu.py)"),
-]
+pytestmark = pytest.mark.gpu
 
 TOL = 1e-4
 
