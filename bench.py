@@ -86,6 +86,21 @@ def cpu_baseline(N, nf, target_s, chunk):
                       f"F+X in the reference's arithmetic, {dt:.1f} s on {threads} threads"}
 
 
+def measured_traffic(N, nf, chunk):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary of this
+    workload (FETCH_SIZE and WRITE_SIZE, calibrated on a copy of known size);
+    empty when there is none for this geometry."""
+    path = os.path.join(ROOT, "profiles", f"fx_fp32_{N}in_nf{nf}_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except OSError:
+        return {}
+    if t.get("workload") != f"{N} inputs, n_freq {nf}, {chunk} frames per launch":
+        return {}
+    return t
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -169,13 +184,15 @@ def main():
                          "frac_fp32": round(x_tflops / FP32_PEAK_TFLOPS, 4),
                          "GB/s": round(x_gbs, 1)},
     }
+    traffic = measured_traffic(N, nf, chunk)
     if f_ms >= x_ms:
         roof = {"bound": "hbm", "achieved": round(f_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(f_gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "fengine_fp32"}
+                "frac": round(f_gbs / HBM_PEAK_GBS, 4), "traffic": traffic.get("fengine_fp32"),
+                "kernel": "fengine_fp32"}
     else:
         roof = {"bound": "mfma", "achieved": round(x_tflops, 2), "peak": FP32_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(x_tflops / FP32_PEAK_TFLOPS, 4), "traffic": None,
-                "kernel": "xengine_fp32"}
+                "unit": "TFLOP/s", "frac": round(x_tflops / FP32_PEAK_TFLOPS, 4),
+                "traffic": traffic.get("xengine_fp32"), "kernel": "xengine_fp32"}
 
     cpu = None
     if args.cpu_seconds > 0:
