@@ -13,10 +13,13 @@ CSRC = os.path.join(_HERE, "csrc")
 LIB_DIR = os.path.join(_HERE, "lib")
 
 
-def build(jobs=8):
+def build(jobs=8, always=True):
     """Compile every HIP/C++ source for gfx950 into lib/ (hipcc cross-compiles
-    without a GPU)."""
-    subprocess.run(["make", "-s", f"-j{jobs}", "-C", CSRC], check=True)
+    without a GPU); `always` recompiles even what is up to date."""
+    cmd = ["make", "-s", f"-j{jobs}", "-C", CSRC]
+    if always:
+        cmd.insert(1, "-B")
+    subprocess.run(cmd, check=True)
 
 
 from . import corr  # noqa: E402,F401

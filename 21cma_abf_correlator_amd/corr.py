@@ -88,7 +88,8 @@ ABI_SYMBOLS = (
 )
 ENGINE_SYMBOLS = (
     "corrx_engine_create", "corrx_engine_destroy", "corrx_fengine", "corrx_xengine",
-    "corrx_scale", "corrx_spectra",
+    "corrx_scale", "corrx_spectra", "corrx_integrator_bytes", "corrx_dump", "corrx_calibrate",
+    "corrx_set_gains", "corrx_gains",
 )
 
 _lib = None
@@ -166,6 +167,11 @@ def declare(L):
         "corrx_xengine": ([vp, vp, i32, vp], st),
         "corrx_scale": ([vp, sz, u64, vp], st),
         "corrx_spectra": ([vp, ctypes.POINTER(vp), ctypes.POINTER(sz)], st),
+        "corrx_integrator_bytes": ([vp, ctypes.POINTER(sz)], st),
+        "corrx_dump": ([vp, vp, u64, vp, vp], st),
+        "corrx_calibrate": ([vp, vp, sz, vp, sz, i32, i32, vp], st),
+        "corrx_set_gains": ([vp, vp, vp], st),
+        "corrx_gains": ([vp, ctypes.POINTER(vp)], st),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -356,8 +362,8 @@ class Correlator:
 class Engine:
     """Engine-level stages on device buffers (include/corrx.h)."""
 
-    def __init__(self, n_station, n_freq, chunk_frames, form=FORM_FP32):
-        self._lib = load()
+    def __init__(self, n_station, n_freq, chunk_frames, form=FORM_FP32, lib=None):
+        self._lib = lib if lib is not None else load()
         sh = corrx_shape(n_station, n_freq, chunk_frames, form)
         h = ctypes.c_void_p()
         check(self._lib.corrx_engine_create(ctypes.byref(sh), ctypes.byref(h)),
@@ -390,6 +396,28 @@ class Engine:
         s = ctypes.c_size_t()
         check(self._lib.corrx_spectra(self.h, ctypes.byref(p), ctypes.byref(s)), "corrx_spectra")
         return p.value, s.value
+
+    def integrator_bytes(self):
+        """Bytes of one integrator element: 8 (FP32), 16 (FP16), 8 (INT8)."""
+        n = ctypes.c_size_t()
+        check(self._lib.corrx_integrator_bytes(self.h, ctypes.byref(n)), "corrx_integrator_bytes")
+        return n.value
+
+    def dump(self, d_acc, frames, d_out, stream=None):
+        check(self._lib.corrx_dump(self.h, d_acc, frames, d_out, stream), "corrx_dump")
+
+    def calibrate(self, d_in, station_pitch, frames, d_mask=None, mask_pitch=0, mask_bit0=0,
+                  stream=None):
+        check(self._lib.corrx_calibrate(self.h, d_in, station_pitch, d_mask, mask_pitch,
+                                        mask_bit0, frames, stream), "corrx_calibrate")
+
+    def set_gains(self, d_gains, stream=None):
+        check(self._lib.corrx_set_gains(self.h, d_gains, stream), "corrx_set_gains")
+
+    def gains(self):
+        p = ctypes.c_void_p()
+        check(self._lib.corrx_gains(self.h, ctypes.byref(p)), "corrx_gains")
+        return p.value
 
 
 def scale(d_acc, n_complex, frames, stream=None):

@@ -43,7 +43,41 @@ extern "C" corr_status corrx_fengine(void *engine, const int8_t *d_in, size_t st
 
 extern "C" corr_status corrx_xengine(void *engine, void *d_acc, int32_t frames, void *stream) {
     if (engine == nullptr) return fail(CORR_ERR_ARG, "engine is NULL");
-    return as_engine(engine)->xengine_fp32(static_cast<float2 *>(d_acc), frames, as_stream(stream));
+    return as_engine(engine)->xengine(d_acc, frames, as_stream(stream));
+}
+
+extern "C" corr_status corrx_integrator_bytes(void *engine, size_t *bytes) {
+    if (engine == nullptr) return fail(CORR_ERR_ARG, "engine is NULL");
+    if (bytes == nullptr) return fail(CORR_ERR_ARG, "bytes is NULL");
+    *bytes = as_engine(engine)->integrator_elem_bytes();
+    return CORR_OK;
+}
+
+extern "C" corr_status corrx_dump(void *engine, const void *d_acc, uint64_t frames, void *d_out,
+                                  void *stream) {
+    if (engine == nullptr) return fail(CORR_ERR_ARG, "engine is NULL");
+    return as_engine(engine)->dump(d_acc, frames, static_cast<float2 *>(d_out), as_stream(stream));
+}
+
+extern "C" corr_status corrx_calibrate(void *engine, const int8_t *d_in, size_t station_pitch,
+                                       const uint8_t *d_mask, size_t mask_pitch, int32_t mask_bit0,
+                                       int32_t frames, void *stream) {
+    if (engine == nullptr) return fail(CORR_ERR_ARG, "engine is NULL");
+    return as_engine(engine)->calibrate(d_in, station_pitch, d_mask, mask_pitch, mask_bit0, frames,
+                                        as_stream(stream));
+}
+
+extern "C" corr_status corrx_set_gains(void *engine, const float *d_gains, void *stream) {
+    if (engine == nullptr) return fail(CORR_ERR_ARG, "engine is NULL");
+    return as_engine(engine)->set_gains(d_gains, as_stream(stream));
+}
+
+extern "C" corr_status corrx_gains(void *engine, const float **d_gains) {
+    if (engine == nullptr) return fail(CORR_ERR_ARG, "engine is NULL");
+    if (d_gains == nullptr) return fail(CORR_ERR_ARG, "output pointer is NULL");
+    if (as_engine(engine)->gains() == nullptr) return fail(CORR_ERR_ARG, "the FP32 form has no gains");
+    *d_gains = as_engine(engine)->gains();
+    return CORR_OK;
 }
 
 extern "C" corr_status corrx_scale(void *d_acc, size_t n_complex, uint64_t frames, void *stream) {

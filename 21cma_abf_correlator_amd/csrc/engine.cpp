@@ -8,7 +8,9 @@
 
 #include <cstdarg>
 #include <cstdio>
+#include <cstring>
 #include <new>
+#include <vector>
 
 namespace corr {
 
@@ -58,8 +60,11 @@ corr_status Engine::create(const EngineShape &shape, Engine **out) {
                      "n_freq=%d: this build transforms power-of-two n_freq in [512, 16384] "
                      "only (FFT length 2*n_freq)",
                      shape.n_freq);
-    if (shape.form != Form::kFP32)
-        return fail(CORR_ERR_NOT_IMPL, "engine: only the FP32 form is available in this build");
+    if (shape.form != Form::kFP32 && shape.form != Form::kFP16 && shape.form != Form::kINT8)
+        return failf(CORR_ERR_ARG, "engine: unknown form %d", (int)shape.form);
+    if (shape.form != Form::kFP32 && shape.n_station > kMaxStationMfma)
+        return failf(CORR_ERR_NOT_IMPL, "engine: the FP16/int8 forms take at most %d inputs (got %d)",
+                     kMaxStationMfma, shape.n_station);
     Engine *e = new (std::nothrow) Engine();
     if (e == nullptr) return fail(CORR_ERR_NOMEM, "engine: out of host memory");
     e->shape_ = shape;
@@ -84,8 +89,14 @@ corr_status Engine::init() {
         (st = hip_check(hipMalloc(reinterpret_cast<void **>(&d_wl_), sizeof(float2) * nf),
                         "hipMalloc(r2c twiddles)")) != CORR_OK)
         return st;
-    const size_t spec_bytes =
-        (size_t)N * (size_t)shape_.chunk_frames * (size_t)nf * sizeof(float2);
+    // The spectrum buffer also holds cal_frames_ frames of complex64 spectra
+    // during calibration.
+    const size_t elem = spectra_elem_bytes();
+    cal_frames_ = (int)((size_t)shape_.chunk_frames * elem / sizeof(float2));
+    if (cal_frames_ < 1) cal_frames_ = 1;
+    size_t per_station = (size_t)shape_.chunk_frames * elem;
+    if (per_station < (size_t)cal_frames_ * sizeof(float2)) per_station = (size_t)cal_frames_ * sizeof(float2);
+    const size_t spec_bytes = (size_t)N * (size_t)nf * per_station;
     if ((st = hip_check(hipMalloc(&d_spec_, spec_bytes), "hipMalloc(spectra)")) != CORR_OK)
         return st;
     if ((st = hip_check(hipMalloc(reinterpret_cast<void **>(&d_valid_),
@@ -108,6 +119,36 @@ corr_status Engine::init() {
                         "hipMemcpy(block pairs)")) != CORR_OK)
         return st;
 
+    // Baseline table: b -> (j, k), j <= k, j outer (reference corr_cuda.cpp:193-204).
+    std::vector<int2> pairs;
+    pairs.reserve((size_t)n_baseline_);
+    for (int j = 0; j < N; ++j)
+        for (int k = j; k < N; ++k) pairs.push_back(make_int2(j, k));
+    if ((st = hip_check(hipMalloc(reinterpret_cast<void **>(&d_pairs_), sizeof(int2) * pairs.size()),
+                        "hipMalloc(baseline pairs)")) != CORR_OK ||
+        (st = hip_check(hipMemcpy(d_pairs_, pairs.data(), sizeof(int2) * pairs.size(), hipMemcpyHostToDevice),
+                        "hipMemcpy(baseline pairs)")) != CORR_OK)
+        return st;
+
+    if (shape_.form != Form::kFP32) {
+        const size_t g = (size_t)N * (size_t)nf;
+        if ((st = hip_check(hipMalloc(reinterpret_cast<void **>(&d_gain_), sizeof(float) * g),
+                            "hipMalloc(gains)")) != CORR_OK ||
+            (st = hip_check(hipMalloc(reinterpret_cast<void **>(&d_power_), sizeof(float) * g),
+                            "hipMalloc(power)")) != CORR_OK ||
+            (st = hip_check(hipMalloc(reinterpret_cast<void **>(&d_count_), sizeof(int32_t) * (size_t)N),
+                            "hipMalloc(counts)")) != CORR_OK)
+            return st;
+        const float one = 1.0f;
+        uint32_t one_bits;
+        std::memcpy(&one_bits, &one, 4);
+        if ((st = hip_check(launch_fill(d_gain_, g, one_bits, nullptr), "fill_kernel(gains)")) != CORR_OK)
+            return st;
+        if ((st = hip_check(xengine_mfma_prepare((int)shape_.form, N), "hipFuncSetAttribute(xengine_mfma)")) !=
+            CORR_OK)
+            return st;
+    }
+
     if ((st = hip_check(launch_twiddles(d_tw_, d_wl_, nf, nullptr), "twiddle_kernel")) != CORR_OK)
         return st;
     return hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize(twiddles)");
@@ -119,6 +160,26 @@ Engine::~Engine() {
     hipFree(d_spec_);
     hipFree(d_valid_);
     hipFree(d_bpairs_);
+    hipFree(d_pairs_);
+    hipFree(d_gain_);
+    hipFree(d_power_);
+    hipFree(d_count_);
+}
+
+size_t Engine::spectra_elem_bytes() const {
+    switch (shape_.form) {
+    case Form::kFP16: return 4;
+    case Form::kINT8: return 2;
+    default: return sizeof(float2);
+    }
+}
+
+size_t Engine::integrator_elem_bytes() const {
+    switch (shape_.form) {
+    case Form::kFP16: return 16;
+    case Form::kINT8: return 8;
+    default: return sizeof(float2);
+    }
 }
 
 corr_status Engine::fengine(const int8_t *d_in, size_t station_pitch, const uint8_t *d_mask,
@@ -143,10 +204,80 @@ corr_status Engine::fengine(const int8_t *d_in, size_t station_pitch, const uint
         if (st != CORR_OK) return st;
         valid = d_valid_;
     }
-    return hip_check(launch_fengine_fp32(log2_nf_, d_in, station_pitch, valid, valid_stride, d_tw_,
-                                         d_wl_, static_cast<float2 *>(d_spec_),
+    if (shape_.form == Form::kFP32)
+        return hip_check(launch_fengine_fp32(log2_nf_, d_in, station_pitch, valid, valid_stride, d_tw_,
+                                             d_wl_, static_cast<float2 *>(d_spec_),
+                                             spectra_station_stride(), frames, shape_.n_station, s),
+                         "fengine_fp32_kernel");
+    return hip_check(launch_fengine_form((int)shape_.form, log2_nf_, d_in, station_pitch, valid,
+                                         valid_stride, d_tw_, d_wl_, d_gain_, d_spec_,
                                          spectra_station_stride(), frames, shape_.n_station, s),
-                     "fengine_fp32_kernel");
+                     "fengine_form_kernel");
+}
+
+corr_status Engine::xengine(void *integ, int frames, hipStream_t s) {
+    if (shape_.form == Form::kFP32) return xengine_fp32(static_cast<float2 *>(integ), frames, s);
+    if (integ == nullptr) return fail(CORR_ERR_ARG, "xengine: integrator is NULL");
+    if (frames < 0 || frames > shape_.chunk_frames)
+        return failf(CORR_ERR_ARG, "xengine: frames=%d outside [0, %d]", frames, shape_.chunk_frames);
+    return hip_check(launch_xengine_mfma((int)shape_.form, shape_.n_station, d_spec_,
+                                         spectra_station_stride(), shape_.n_freq, frames, integ, s),
+                     "xengine_mfma_kernel");
+}
+
+corr_status Engine::dump(const void *integ, uint64_t frames, float2 *out, hipStream_t s) {
+    if (integ == nullptr || out == nullptr) return fail(CORR_ERR_ARG, "dump: NULL buffer");
+    if (frames == 0) return fail(CORR_ERR_ARG, "dump: frames must be > 0");
+    return hip_check(launch_dump((int)shape_.form, integ, n_baseline_, shape_.n_freq, d_pairs_, d_gain_,
+                                 frames, out, s),
+                     "dump_kernel");
+}
+
+corr_status Engine::calibrate(const int8_t *d_in, size_t station_pitch, const uint8_t *d_mask,
+                              size_t mask_pitch, int mask_bit0, int frames, hipStream_t s) {
+    if (shape_.form == Form::kFP32) return fail(CORR_ERR_ARG, "calibrate: the FP32 form has no gains");
+    if (frames < 1) return fail(CORR_ERR_ARG, "calibrate: frames must be >= 1");
+    if (d_in == nullptr) return fail(CORR_ERR_ARG, "calibrate: input is NULL");
+    const int N = shape_.n_station, nf = shape_.n_freq;
+    if (station_pitch < (size_t)frames * 2u * (size_t)nf)
+        return fail(CORR_ERR_ARG, "calibrate: station pitch shorter than the frames it holds");
+    if (mask_bit0 < 0) return fail(CORR_ERR_ARG, "calibrate: negative mask bit offset");
+    if (d_mask != nullptr && mask_pitch * 8u < (size_t)mask_bit0 + (size_t)frames)
+        return fail(CORR_ERR_ARG, "calibrate: mask pitch shorter than the bits it must hold");
+    const size_t g = (size_t)N * (size_t)nf;
+    corr_status st;
+    if ((st = hip_check(launch_fill(d_power_, g, 0u, s), "fill_kernel(power)")) != CORR_OK ||
+        (st = hip_check(launch_fill(d_count_, (size_t)N, 0u, s), "fill_kernel(counts)")) != CORR_OK)
+        return st;
+    const size_t valid_stride = (size_t)shape_.chunk_frames;
+    const size_t cal_stride = (size_t)cal_frames_ * (size_t)nf;
+    float2 *spec = static_cast<float2 *>(d_spec_);
+    for (int b0 = 0; b0 < frames; b0 += cal_frames_) {
+        const int nb = frames - b0 < cal_frames_ ? frames - b0 : cal_frames_;
+        const int32_t *valid = nullptr;
+        if (d_mask != nullptr) {
+            if ((st = hip_check(launch_validity(d_mask, mask_pitch, mask_bit0 + b0, nb, N, d_valid_,
+                                                valid_stride, s),
+                                "validity_kernel")) != CORR_OK)
+                return st;
+            valid = d_valid_;
+        }
+        if ((st = hip_check(launch_fengine_fp32(log2_nf_, d_in + (size_t)b0 * 2u * (size_t)nf, station_pitch,
+                                                valid, valid_stride, d_tw_, d_wl_, spec, cal_stride, nb, N, s),
+                            "fengine_fp32_kernel")) != CORR_OK ||
+            (st = hip_check(launch_power(nf, nb, N, spec, cal_stride, valid, valid_stride, d_power_, d_count_, s),
+                            "power_kernel")) != CORR_OK)
+            return st;
+    }
+    return hip_check(launch_gains((int)shape_.form, nf, N, d_power_, d_count_, d_gain_, s), "gains_kernel");
+}
+
+corr_status Engine::set_gains(const float *d_gains, hipStream_t s) {
+    if (shape_.form == Form::kFP32) return fail(CORR_ERR_ARG, "set_gains: the FP32 form has no gains");
+    if (d_gains == nullptr) return fail(CORR_ERR_ARG, "set_gains: gains are NULL");
+    return hip_check(hipMemcpyAsync(d_gain_, d_gains, sizeof(float) * (size_t)shape_.n_station * shape_.n_freq,
+                                    hipMemcpyDeviceToDevice, s),
+                     "hipMemcpyAsync(gains)");
 }
 
 corr_status Engine::xengine_fp32(float2 *acc, int frames, hipStream_t s) {

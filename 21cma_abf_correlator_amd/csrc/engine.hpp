@@ -11,6 +11,9 @@
 //               dropped) + the form's output conversion, in one kernel
 //   xengine   : acc[b][c] += sum over the chunk's frames of Y_j conj(Y_k)
 //   scale     : acc *= 1.0f/frames (reference kernels.cu:178-185, 890-891)
+//   dump      : integrator -> complex64 [baseline][n_freq] visibilities
+// and, for the FP16 and int8 forms,
+//   calibrate : per-(input, channel) gains from the mean power of a block of frames
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -39,6 +42,9 @@ enum class Form : int {
 /// Largest station count the drop-in ABI accepts on one device (1024 baselines,
 /// reference kernels.cu:108-110, 474-483).
 constexpr int kMaxBaselinesAbi = 1024;
+
+/// Largest input count of the FP16/int8 forms (2*n_station <= 96 Gram rows).
+constexpr int kMaxStationMfma = 48;
 
 /// FFT sizes this build transforms: n_freq = 2^k with 512 <= n_freq <= 16384.
 bool fft_size_supported(int n_freq);
@@ -70,12 +76,36 @@ class Engine {
     /// acc[b*n_freq + c] (complex64) += sum over the last fengine's frames.
     corr_status xengine_fp32(float2 *acc, int frames, hipStream_t s);
 
+    /// The form's X-engine: integ[b*n_freq + c] += sum over the last fengine's
+    /// frames; integrator element float2 (FP32), float4 = (sum re, sum im,
+    /// compensation re, compensation im) (FP16), int2 (int8).
+    corr_status xengine(void *integ, int frames, hipStream_t s);
+
+    /// Bytes of one integrator element of this form.
+    size_t integrator_elem_bytes() const;
+
+    /// Visibilities of an integration of `frames` frames (the caller's nominal
+    /// count): out[b*n_freq + c] complex64; FP32 form as the reference's row
+    /// scale, FP16/int8 forms rescaled by 1/(g_j g_k frames) in FP64.
+    corr_status dump(const void *integ, uint64_t frames, float2 *out, hipStream_t s);
+
+    /// FP16/int8 forms: gains from the mean power of `frames` frames (same
+    /// arguments as fengine, any number of frames); overwrites the spectra.
+    corr_status calibrate(const int8_t *d_in, size_t station_pitch, const uint8_t *d_mask,
+                          size_t mask_pitch, int mask_bit0, int frames, hipStream_t s);
+
+    /// FP16/int8 forms: gains [station][n_freq] (positive, finite) from device memory.
+    corr_status set_gains(const float *d_gains, hipStream_t s);
+    const float *gains() const { return d_gain_; }
+
     /// acc *= 1.0f/(float)frames over `n` complex values.
     static corr_status scale(float2 *acc, size_t n, uint64_t frames, hipStream_t s);
 
-    /// Spectrum buffer: [station][chunk_frames][n_freq] of the form's element.
+    /// Spectrum buffer: [station][chunk_frames][n_freq] of the form's element
+    /// (complex64, packed FP16 pair in 32 bits, packed int8 pair in 16 bits).
     void *spectra() const { return d_spec_; }
     size_t spectra_station_stride() const { return (size_t)shape_.chunk_frames * shape_.n_freq; }
+    size_t spectra_elem_bytes() const;
 
     /// Device-side launch statistics for the last X-engine plan (for reports).
     int xengine_waves() const { return x_waves_; }
@@ -88,10 +118,16 @@ class Engine {
     int n_baseline_ = 0;
     int log2_nf_ = 0;
 
+    int cal_frames_ = 1;        ///< frames per calibration batch (FP32 spectra in d_spec_)
+
     float2 *d_tw_ = nullptr;    ///< exp(-2 pi i m / n_freq), m < n_freq
     float2 *d_wl_ = nullptr;    ///< exp(-2 pi i k / (2 n_freq)), k < n_freq
     void *d_spec_ = nullptr;
     int32_t *d_valid_ = nullptr;  ///< expanded validity, [station][chunk_frames]
+    int2 *d_pairs_ = nullptr;     ///< (j, k) of every baseline
+    float *d_gain_ = nullptr;     ///< FP16/int8 forms: [station][n_freq]
+    float *d_power_ = nullptr;    ///< calibration: [station][n_freq] sum of |Y|^2
+    int32_t *d_count_ = nullptr;  ///< calibration: frames seen per station
 
     // FP32 X-engine plan: 4x4 station blocks, lower-triangle block pairs.
     int2 *d_bpairs_ = nullptr;
@@ -115,5 +151,21 @@ hipError_t launch_xengine_fp32(const float2 *y, size_t y_station_stride, int n_f
                                int npair, int waves, int groups, int frames_lds, float2 *acc,
                                hipStream_t s);
 hipError_t launch_scale(float2 *acc, size_t n, float s, hipStream_t st);
+hipError_t launch_fengine_form(int form, int log2_nf, const int8_t *in, size_t station_pitch,
+                               const int32_t *valid, size_t valid_stride, const float2 *tw,
+                               const float2 *wl, const float *gain, void *out,
+                               size_t out_station_stride, int frames, int n_station,
+                               hipStream_t s);
+hipError_t launch_power(int nf, int frames, int n_station, const float2 *spec,
+                        size_t spec_station_stride, const int32_t *valid, size_t valid_stride,
+                        float *power, int32_t *count, hipStream_t s);
+hipError_t launch_gains(int form, int nf, int n_station, const float *power, const int32_t *count,
+                        float *gain, hipStream_t s);
+hipError_t launch_fill(void *p, size_t n_words, uint32_t v, hipStream_t s);
+hipError_t xengine_mfma_prepare(int form, int n_station);
+hipError_t launch_xengine_mfma(int form, int n_station, const void *spectra, size_t ys, int nf,
+                               int frames, void *integ, hipStream_t s);
+hipError_t launch_dump(int form, const void *integ, int n_baseline, int nf, const int2 *pairs,
+                       const float *gain, uint64_t frames, float2 *out, hipStream_t s);
 
 }  // namespace corr

@@ -28,12 +28,40 @@
 #endif
 #include "fft_body.hpp"
 #include "xengine_body.hpp"
+#include "xmfma_body.hpp"
 
 namespace corr {
 namespace {
 
 struct DeviceSync {
     __device__ void operator()() const { __syncthreads(); }
+};
+
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef float floatx4_t __attribute__((ext_vector_type(4)));
+typedef int intx4_t __attribute__((ext_vector_type(4)));
+
+// The wave-wide operations of the MFMA X-engine on gfx950.
+struct MfmaOps {
+    // D = A B + C, 16x16x32 FP16 -> FP32
+    __device__ xmfma::Acc<float> mma(const xmfma::Frag &a, const xmfma::Frag &b,
+                                     const xmfma::Acc<float> &c) const {
+        const floatx4_t d = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+            __builtin_bit_cast(half8_t, a), __builtin_bit_cast(half8_t, b),
+            __builtin_bit_cast(floatx4_t, c), 0, 0, 0);
+        return __builtin_bit_cast(xmfma::Acc<float>, d);
+    }
+    // D = A B + C, 16x16x64 int8 -> int32
+    __device__ xmfma::Acc<int32_t> mma(const xmfma::Frag &a, const xmfma::Frag &b,
+                                       const xmfma::Acc<int32_t> &c) const {
+        const intx4_t d = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+            __builtin_bit_cast(intx4_t, a), __builtin_bit_cast(intx4_t, b),
+            __builtin_bit_cast(intx4_t, c), 0, 0, 0);
+        return __builtin_bit_cast(xmfma::Acc<int32_t>, d);
+    }
+    // value of lane l^1
+    __device__ float xor1(float x) const { return __shfl_xor(x, 1); }
+    __device__ int32_t xor1(int32_t x) const { return __shfl_xor(x, 1); }
 };
 
 // ---------------------------------------------------------------------------
@@ -51,6 +79,22 @@ fengine_fp32_kernel(const int8_t *__restrict__ in, size_t station_pitch,
     fft::fengine_fp32_block<LOG2N>((int)threadIdx.x, (int)blockIdx.x, (int)blockIdx.y, lds, sync, in,
                                    station_pitch, valid, valid_stride, tw, wl, out,
                                    out_station_stride);
+}
+
+// F-engine of the FP16 (FORM 1) and int8 (FORM 2) forms: as above, then gain
+// and conversion per bin; output [station][frame][N] packed complex.
+template <int LOG2N, int FORM>
+__global__ void __launch_bounds__((1 << LOG2N) / 16)
+fengine_form_kernel(const int8_t *__restrict__ in, size_t station_pitch,
+                    const int32_t *__restrict__ valid, size_t valid_stride,
+                    const float2 *__restrict__ tw, const float2 *__restrict__ wl,
+                    const float *__restrict__ gain, void *__restrict__ out,
+                    size_t out_station_stride) {
+    __shared__ float2 lds[fft::Sizes<(1 << LOG2N)>::kLds];
+    DeviceSync sync;
+    fft::fengine_form_block<LOG2N, FORM>((int)threadIdx.x, (int)blockIdx.x, (int)blockIdx.y, lds,
+                                         sync, in, station_pitch, valid, valid_stride, tw, wl, gain,
+                                         out, out_station_stride);
 }
 
 // Validity bits -> int32 per (station, frame); grid (ceil(frames/256), stations).
@@ -100,6 +144,153 @@ __global__ void __launch_bounds__(kScaleBlock) scale_kernel(float2 *acc, size_t 
         v.x *= s;
         v.y *= s;
         acc[i] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// MFMA X-engine (FP16 / int8 forms)
+// grid (n_freq/16), block 512; dynamic LDS xmfma::lds_bytes(n_station)
+// ---------------------------------------------------------------------------
+template <int FORM, int NRT>
+__global__ void __launch_bounds__(xmfma::kThreads)
+xengine_mfma_kernel(const void *__restrict__ spectra, size_t ys, int nf, int frames, int n_station,
+                    int n_tiles, void *__restrict__ integ) {
+    extern __shared__ uint4 lds_words[];
+    DeviceSync sync;
+    MfmaOps ops;
+    xmfma::xengine_mfma_block<FORM, NRT>((int)threadIdx.x, (int)blockIdx.x, n_tiles,
+                                         reinterpret_cast<uint8_t *>(lds_words), sync, ops, spectra,
+                                         ys, nf, frames, n_station, integ);
+}
+
+// Calibration power sum and gains; grid (ceil(nf/256), stations), block 256.
+constexpr unsigned kChanBlock = 256;
+__global__ void __launch_bounds__(kChanBlock)
+power_kernel(int nf, int frames, const float2 *__restrict__ spec, size_t spec_station_stride,
+             const int32_t *__restrict__ valid, size_t valid_stride, float *__restrict__ power,
+             int32_t *__restrict__ count) {
+    fft::power_sum((int)(blockIdx.x * kChanBlock + threadIdx.x), (int)blockIdx.y, nf, frames, spec,
+                   spec_station_stride, valid, valid_stride, power, count);
+}
+
+__global__ void __launch_bounds__(kChanBlock)
+gains_kernel(int form, int nf, const float *__restrict__ power, const int32_t *__restrict__ count,
+             float *__restrict__ gain) {
+    const int c = (int)(blockIdx.x * kChanBlock + threadIdx.x), st = (int)blockIdx.y;
+    if (c >= nf) return;
+    const size_t at = (size_t)st * (size_t)nf + (size_t)c;
+    gain[at] = fft::gain_from_power(form, power[at], count[st]);
+}
+
+// Fill n 32-bit words with v (grid-stride over `total` threads, block 256).
+__global__ void __launch_bounds__(kChanBlock) fill_kernel(uint32_t *p, size_t n, uint32_t v, size_t total) {
+    for (size_t i = (size_t)blockIdx.x * kChanBlock + threadIdx.x; i < n; i += total) p[i] = v;
+}
+
+// Dump of one integration into complex64 [baseline][nf]; grid (ceil(nf/256), baselines).
+//   FORM 0: acc * (1.0f/frames) in FP32 (the reference's row scale)
+//   FORM 1: ((double)sum + (double)compensation) / (g_j g_k frames) in FP64
+//   FORM 2: (double)int32 / (g_j g_k frames) in FP64
+template <int FORM>
+__global__ void __launch_bounds__(kChanBlock)
+dump_kernel(const void *__restrict__ integ, int nf, const int2 *__restrict__ pairs,
+            const float *__restrict__ gain, uint64_t frames, float inv_frames_f32,
+            float2 *__restrict__ out) {
+    const int c = (int)(blockIdx.x * kChanBlock + threadIdx.x), b = (int)blockIdx.y;
+    if (c >= nf) return;
+    const size_t at = (size_t)b * (size_t)nf + (size_t)c;
+    if constexpr (FORM == 0) {
+        const float2 v = static_cast<const float2 *>(integ)[at];
+        out[at] = make_float2(v.x * inv_frames_f32, v.y * inv_frames_f32);
+    } else {
+        const int2 jk = pairs[b];
+        const double den = (double)gain[(size_t)jk.x * nf + c] * (double)gain[(size_t)jk.y * nf + c] *
+                           (double)frames;
+        double re, im;
+        if constexpr (FORM == 1) {
+            const float4 v = static_cast<const float4 *>(integ)[at];
+            re = ((double)v.x + (double)v.z) / den;
+            im = ((double)v.y + (double)v.w) / den;
+        } else {
+            const int2 v = static_cast<const int2 *>(integ)[at];
+            re = (double)v.x / den;
+            im = (double)v.y / den;
+        }
+        out[at] = make_float2((float)re, (float)im);
+    }
+}
+
+template <int LOG2N, int FORM>
+hipError_t launch_fengine_form_t(const int8_t *in, size_t station_pitch, const int32_t *valid,
+                                 size_t valid_stride, const float2 *tw, const float2 *wl,
+                                 const float *gain, void *out, size_t out_station_stride,
+                                 int frames, int n_station, hipStream_t s) {
+    const dim3 grid((unsigned)frames, (unsigned)n_station);
+    const dim3 block((1u << LOG2N) / 16u);
+    hipLaunchKernelGGL((fengine_form_kernel<LOG2N, FORM>), grid, block, 0, s, in, station_pitch,
+                       valid, valid_stride, tw, wl, gain, out, out_station_stride);
+    return hipGetLastError();
+}
+
+template <int FORM>
+hipError_t launch_fengine_form_f(int log2_nf, const int8_t *in, size_t station_pitch,
+                                 const int32_t *valid, size_t valid_stride, const float2 *tw,
+                                 const float2 *wl, const float *gain, void *out,
+                                 size_t out_station_stride, int frames, int n_station,
+                                 hipStream_t s) {
+    switch (log2_nf) {
+#define CORR_FFORM_CASE(L)                                                                       \
+    case L:                                                                                      \
+        return launch_fengine_form_t<L, FORM>(in, station_pitch, valid, valid_stride, tw, wl,   \
+                                              gain, out, out_station_stride, frames, n_station, \
+                                              s);
+        CORR_FFORM_CASE(9)
+        CORR_FFORM_CASE(10)
+        CORR_FFORM_CASE(11)
+        CORR_FFORM_CASE(12)
+        CORR_FFORM_CASE(13)
+        CORR_FFORM_CASE(14)
+#undef CORR_FFORM_CASE
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
+template <int FORM>
+const void *xengine_mfma_fn(int nrt) {
+    switch (nrt) {
+    case 1: return reinterpret_cast<const void *>(&xengine_mfma_kernel<FORM, 1>);
+    case 2: return reinterpret_cast<const void *>(&xengine_mfma_kernel<FORM, 2>);
+    case 3: return reinterpret_cast<const void *>(&xengine_mfma_kernel<FORM, 3>);
+    case 4: return reinterpret_cast<const void *>(&xengine_mfma_kernel<FORM, 4>);
+    case 5: return reinterpret_cast<const void *>(&xengine_mfma_kernel<FORM, 5>);
+    case 6: return reinterpret_cast<const void *>(&xengine_mfma_kernel<FORM, 6>);
+    default: return nullptr;
+    }
+}
+
+template <int FORM>
+hipError_t launch_xengine_mfma_f(int n_station, const void *spectra, size_t ys, int nf, int frames,
+                                 void *integ, hipStream_t s) {
+    const int nrt = xmfma::row_tiles(n_station);
+    const int n_tiles = nf / xmfma::kChannels;
+    const size_t shm = xmfma::lds_bytes(n_station);
+    const dim3 grid((unsigned)n_tiles), block((unsigned)xmfma::kThreads);
+    switch (nrt) {
+#define CORR_XMFMA_CASE(R)                                                                          \
+    case R:                                                                                         \
+        hipLaunchKernelGGL((xengine_mfma_kernel<FORM, R>), grid, block, shm, s, spectra, ys, nf,   \
+                           frames, n_station, n_tiles, integ);                                      \
+        return hipGetLastError();
+        CORR_XMFMA_CASE(1)
+        CORR_XMFMA_CASE(2)
+        CORR_XMFMA_CASE(3)
+        CORR_XMFMA_CASE(4)
+        CORR_XMFMA_CASE(5)
+        CORR_XMFMA_CASE(6)
+#undef CORR_XMFMA_CASE
+    default:
+        return hipErrorInvalidValue;
     }
 }
 
@@ -176,6 +367,89 @@ hipError_t launch_xengine_fp32(const float2 *y, size_t y_station_stride, int n_f
     default:
         return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_fengine_form(int form, int log2_nf, const int8_t *in, size_t station_pitch,
+                               const int32_t *valid, size_t valid_stride, const float2 *tw,
+                               const float2 *wl, const float *gain, void *out,
+                               size_t out_station_stride, int frames, int n_station,
+                               hipStream_t s) {
+    if (frames <= 0 || n_station <= 0) return hipSuccess;
+    if (form == 1)
+        return launch_fengine_form_f<1>(log2_nf, in, station_pitch, valid, valid_stride, tw, wl, gain,
+                                        out, out_station_stride, frames, n_station, s);
+    if (form == 2)
+        return launch_fengine_form_f<2>(log2_nf, in, station_pitch, valid, valid_stride, tw, wl, gain,
+                                        out, out_station_stride, frames, n_station, s);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_power(int nf, int frames, int n_station, const float2 *spec,
+                        size_t spec_station_stride, const int32_t *valid, size_t valid_stride,
+                        float *power, int32_t *count, hipStream_t s) {
+    if (frames <= 0 || n_station <= 0 || nf <= 0) return hipSuccess;
+    const dim3 grid((unsigned)((nf + kChanBlock - 1) / kChanBlock), (unsigned)n_station);
+    hipLaunchKernelGGL(power_kernel, grid, dim3(kChanBlock), 0, s, nf, frames, spec,
+                       spec_station_stride, valid, valid_stride, power, count);
+    return hipGetLastError();
+}
+
+hipError_t launch_gains(int form, int nf, int n_station, const float *power, const int32_t *count,
+                        float *gain, hipStream_t s) {
+    if (n_station <= 0 || nf <= 0) return hipSuccess;
+    const dim3 grid((unsigned)((nf + kChanBlock - 1) / kChanBlock), (unsigned)n_station);
+    hipLaunchKernelGGL(gains_kernel, grid, dim3(kChanBlock), 0, s, form, nf, power, count, gain);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(void *p, size_t n_words, uint32_t v, hipStream_t s) {
+    if (n_words == 0) return hipSuccess;
+    size_t blocks = (n_words + kChanBlock - 1) / kChanBlock;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(fill_kernel, dim3((unsigned)blocks), dim3(kChanBlock), 0, s,
+                       static_cast<uint32_t *>(p), n_words, v, blocks * kChanBlock);
+    return hipGetLastError();
+}
+
+hipError_t xengine_mfma_prepare(int form, int n_station) {
+    const int nrt = xmfma::row_tiles(n_station);
+    const void *fn = form == 1 ? xengine_mfma_fn<1>(nrt) : form == 2 ? xengine_mfma_fn<2>(nrt) : nullptr;
+    if (fn == nullptr) return hipErrorInvalidValue;
+    return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)xmfma::lds_bytes(n_station));
+}
+
+hipError_t launch_xengine_mfma(int form, int n_station, const void *spectra, size_t ys, int nf,
+                               int frames, void *integ, hipStream_t s) {
+    if (frames <= 0) return hipSuccess;
+    if (nf % xmfma::kChannels != 0) return hipErrorInvalidValue;
+    if (form == 1) return launch_xengine_mfma_f<1>(n_station, spectra, ys, nf, frames, integ, s);
+    if (form == 2) return launch_xengine_mfma_f<2>(n_station, spectra, ys, nf, frames, integ, s);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_dump(int form, const void *integ, int n_baseline, int nf, const int2 *pairs,
+                       const float *gain, uint64_t frames, float2 *out, hipStream_t s) {
+    if (n_baseline <= 0 || nf <= 0) return hipSuccess;
+    const dim3 grid((unsigned)((nf + kChanBlock - 1) / kChanBlock), (unsigned)n_baseline);
+    const float inv = 1.0f / (float)frames;
+    switch (form) {
+    case 0:
+        hipLaunchKernelGGL(dump_kernel<0>, grid, dim3(kChanBlock), 0, s, integ, nf, pairs, gain,
+                           frames, inv, out);
+        break;
+    case 1:
+        hipLaunchKernelGGL(dump_kernel<1>, grid, dim3(kChanBlock), 0, s, integ, nf, pairs, gain,
+                           frames, inv, out);
+        break;
+    case 2:
+        hipLaunchKernelGGL(dump_kernel<2>, grid, dim3(kChanBlock), 0, s, integ, nf, pairs, gain,
+                           frames, inv, out);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_scale(float2 *acc, size_t n, float s, hipStream_t st) {

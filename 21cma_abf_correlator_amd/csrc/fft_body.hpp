@@ -18,6 +18,8 @@
 #include <cstddef>
 #include <cstdint>
 
+#include "forms.hpp"
+
 namespace corr {
 namespace fft {
 
@@ -242,6 +244,74 @@ CORR_HD inline void fengine_fp32_block(int j, int f, int st, float2 *lds, Sync &
         const int k = j + q * NT;
         dst[k] = r2c_bin<N>(lds, wl, k);
     }
+}
+
+// Whole F-engine workgroup for the FP16 (FORM 1) or int8 (FORM 2) form: bin k
+// of station st is scaled by gain[st*N + k] and converted (forms.hpp); output
+// [station][frame][N] of packed 32-bit (FP16) or 16-bit (int8) complex.
+template <int LOG2N, int FORM, class Sync>
+CORR_HD inline void fengine_form_block(int j, int f, int st, float2 *lds, Sync &sync, const int8_t *in,
+                                       size_t station_pitch, const int32_t *valid, size_t valid_stride,
+                                       const float2 *tw, const float2 *wl, const float *gain,
+                                       void *out, size_t out_station_stride) {
+    constexpr int N = 1 << LOG2N;
+    constexpr int NT = N / 16;
+    complex_fft_block<LOG2N>(j, f, st, lds, sync, in, station_pitch, valid, valid_stride, tw);
+    const float *g = gain + (size_t)st * N;
+    const size_t at = (size_t)st * out_station_stride + (size_t)f * (size_t)N;
+    if constexpr (FORM == 1) {
+        uint32_t *dst = static_cast<uint32_t *>(out) + at;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int k = j + q * NT;
+            dst[k] = forms::pack_fp16(r2c_bin<N>(lds, wl, k), g[k]);
+        }
+    } else {
+        uint16_t *dst = static_cast<uint16_t *>(out) + at;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int k = j + q * NT;
+            dst[k] = forms::pack_int8(r2c_bin<N>(lds, wl, k), g[k]);
+        }
+    }
+}
+
+// Calibration: power[st*nf + c] += sum over the frames present of |Y|^2, in
+// frame order; count[st] += frames present (by the channel-0 thread).
+// Thread c of the workgroup row of station st.
+CORR_HD inline void power_sum(int c, int st, int nf, int frames, const float2 *spec, size_t spec_station_stride,
+                              const int32_t *valid, size_t valid_stride, float *power, int32_t *count) {
+    if (c >= nf) return;
+    const float2 *y = spec + (size_t)st * spec_station_stride + (size_t)c;
+    float p = 0.0f;
+    int n = 0;
+    for (int t = 0; t < frames; ++t) {
+        if (valid != nullptr && valid[(size_t)st * valid_stride + (size_t)t] == 0) continue;
+        const float2 v = y[(size_t)t * (size_t)nf];
+        p += v.x * v.x + v.y * v.y;
+        ++n;
+    }
+    power[(size_t)st * nf + c] += p;
+    if (c == 0) count[st] += n;
+}
+
+// Gain of (station st, channel c) from the calibration power (forms.hpp):
+//   FP16: 2^(8-E) with sigma = m 2^E, m in [0.5, 1): the component RMS lands in [128, 256)
+//   int8: 1/(0.35 sigma)
+// sigma = per-component RMS = sqrt(mean |Y|^2 / 2); gain 1 where nothing was seen.
+CORR_HD inline float gain_from_power(int form, float power, int32_t count) {
+    if (count <= 0 || !(power > 0.0f) || !(power < 3.0e38f)) return 1.0f;
+    const float sigma = sqrtf(power / (float)count * 0.5f);
+    if (!(sigma > 0.0f)) return 1.0f;
+    if (form == 1) {
+        int e = 0;
+        (void)frexpf(sigma, &e);
+        int s = 8 - e;
+        if (s > 60) s = 60;
+        if (s < -60) s = -60;
+        return ldexpf(1.0f, s);
+    }
+    return 1.0f / (forms::kInt8StepSigma * sigma);
 }
 
 }  // namespace fft

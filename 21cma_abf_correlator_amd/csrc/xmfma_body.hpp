@@ -1,0 +1,308 @@
+// xmfma_body.hpp -- the MFMA X-engine of the FP16 and int8 forms, written once for
+// the gfx950 kernel and for a CPU build of the same code (one host thread per
+// lane; the matrix instruction and the lane exchange are provided by an `Ops`
+// object, so the CPU build can emulate them with the same lane layout).
+//
+// Per channel c the chunk's spectra form a real matrix G (rows x frames):
+//   row 2s   = Re Y_s[t][c],  row 2s+1 = Im Y_s[t][c]   (s = input)
+// and the X-engine accumulates the Gram matrix M = G G^T on 16x16 MFMA tiles,
+// lower triangle of tiles only. Every visibility (j <= k) is then a 2x2 block
+// of M (the reference's Re = xr*yr + xi*yi, Im = xi*yr - xr*yi with x = Y_j,
+// y = Y_k; corr-cuda/src/kernels.cu:243-246):
+//   Re V_jk = M[2k][2j] + M[2k+1][2j+1],   Im V_jk = M[2k][2j+1] - M[2k+1][2j]
+// Products of two FP16 values are exact in FP32 and those of two int8 values
+// exact in int32; the chunk's sum is added into the integrator once per call,
+// with Neumaier-compensated FP32 (FP16 form) or exactly in int32 (int8 form).
+//
+// Workgroup: 16 channels, 8 waves, 2 channels per wave. One step stages 32
+// (FP16) or 64 (int8) frames of all inputs for the 16 channels in LDS as
+//   lds[channel][row][frame]  (row pitch 80 B = 64 data + 16 pad,
+//                              channel pitch rows_padded*80 + 16 B)
+// so that a lane's MFMA operand (16 B: 8 halves / 16 int8 of one row) is one
+// 16-byte LDS read. The staging reads 16 B (4 channels, FP16) or 8 B (4
+// channels, int8) per (input, frame) from the [input][frame][channel] spectra
+// and transposes them in registers.
+//
+// MFMA lane layout assumed for the 16x16xK instructions (operands of K/4
+// elements per lane):
+//   A (16 x K): lane l holds row l%16, elements (l/16)*K/4 .. +K/4-1 of K
+//   B (K x 16): lane l holds column l%16, the same K slice
+//   D (16x16):  lane l, register v holds D[4*(l/16) + v][l%16]
+// A and B take the same fragment (both are rows of G), so the K order inside a
+// lane does not matter; the row/column assignment and D's layout do.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+
+namespace corr {
+namespace xmfma {
+
+constexpr int kChannels = 16;       ///< channels per workgroup
+constexpr int kWaves = 8;           ///< waves per workgroup
+constexpr int kThreads = 64 * kWaves;
+constexpr int kChanPerWave = kChannels / kWaves;
+constexpr int kRowBytes = 80;       ///< 64 bytes of one row per step + 16 pad
+constexpr int kMaxRowTiles = 6;     ///< 2*n_station <= 96
+
+/// One lane's MFMA operand: 16 bytes.
+struct Frag {
+    uint32_t w[4];
+};
+
+template <class T>
+struct Acc {
+    T v[4];
+};
+
+/// Form-dependent constants: FP16 (spectrum word = 2 halves) and int8
+/// (spectrum word = 2 int8 in 16 bits).
+template <int FORM>
+struct Traits;
+
+template <>
+struct Traits<1> {  // FP16
+    using acc_t = float;
+    static constexpr int kFramesPerStep = 32;
+    static constexpr int kElemBytes = 4;
+};
+
+template <>
+struct Traits<2> {  // int8
+    using acc_t = int32_t;
+    static constexpr int kFramesPerStep = 64;
+    static constexpr int kElemBytes = 2;
+};
+
+CORR_HD inline int rows_padded(int n_station) { return ((2 * n_station + 15) / 16) * 16; }
+CORR_HD inline int row_tiles(int n_station) { return rows_padded(n_station) / 16; }
+CORR_HD inline size_t chan_pitch(int n_station) { return (size_t)rows_padded(n_station) * kRowBytes + 16; }
+
+/// Dynamic LDS bytes: the staging tile, reused for the [baseline][16] results.
+CORR_HD inline size_t lds_bytes(int n_station) {
+    const size_t stage = (size_t)kChannels * chan_pitch(n_station);
+    const size_t nb = (size_t)n_station * (size_t)(n_station + 1) / 2;
+    const size_t out = nb * kChannels * 8;
+    return stage > out ? stage : out;
+}
+
+/// Workgroup -> channel tile. Consecutive workgroups are spread over the eight
+/// XCDs; this sends neighbouring channel tiles (which share cache lines of the
+/// spectra) to the same XCD, in dispatch order.
+CORR_HD inline int channel_tile(int bx, int n_tiles) {
+    if (n_tiles % 8 != 0) return bx;
+    return (bx % 8) * (n_tiles / 8) + bx / 8;
+}
+
+// Neumaier's compensated sum: (s, e) += x.
+CORR_HD inline void neumaier(float &s, float &e, float x) {
+    const float t = s + x;
+    const float as = s < 0.0f ? -s : s, ax = x < 0.0f ? -x : x;
+    if (as >= ax)
+        e += (s - t) + x;
+    else
+        e += (x - t) + s;
+    s = t;
+}
+
+// Staging of one item: 16 B (FP16) per frame for 8 frames, or 8 B (int8) per
+// frame for 16 frames, of one (input, channel quad); the 8 transposed rows of
+// 16 B go to lds[(4q+m)][2s+p][16*o .. 16*o+15] for channel m of the quad and
+// part p (0 = real, 1 = imaginary).
+template <int FORM>
+CORR_HD inline void stage_item(int item, uint8_t *lds, const void *spectra, size_t ys, int nf,
+                               int frames, int f0, int c0, size_t cpitch);
+
+template <>
+CORR_HD inline void stage_item<1>(int item, uint8_t *lds, const void *spectra, size_t ys, int nf,
+                                  int frames, int f0, int c0, size_t cpitch) {
+    const int q = item & 3, o = (item >> 2) & 3, s = item >> 4;
+    const uint32_t *y = static_cast<const uint32_t *>(spectra);
+    uint32_t w[8][4];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int f = f0 + 8 * o + i;
+        if (f < frames) {
+            const uint32_t *src = y + (size_t)s * ys + (size_t)f * (size_t)nf + (size_t)(c0 + 4 * q);
+            const uint4 v = *reinterpret_cast<const uint4 *>(src);
+            w[i][0] = v.x;
+            w[i][1] = v.y;
+            w[i][2] = v.z;
+            w[i][3] = v.w;
+        } else {
+            w[i][0] = w[i][1] = w[i][2] = w[i][3] = 0u;
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        uint4 re, im;
+        re.x = (w[0][m] & 0xFFFFu) | (w[1][m] << 16);
+        re.y = (w[2][m] & 0xFFFFu) | (w[3][m] << 16);
+        re.z = (w[4][m] & 0xFFFFu) | (w[5][m] << 16);
+        re.w = (w[6][m] & 0xFFFFu) | (w[7][m] << 16);
+        im.x = (w[0][m] >> 16) | (w[1][m] & 0xFFFF0000u);
+        im.y = (w[2][m] >> 16) | (w[3][m] & 0xFFFF0000u);
+        im.z = (w[4][m] >> 16) | (w[5][m] & 0xFFFF0000u);
+        im.w = (w[6][m] >> 16) | (w[7][m] & 0xFFFF0000u);
+        uint8_t *row = lds + (size_t)(4 * q + m) * cpitch + (size_t)(2 * s) * kRowBytes + 16 * o;
+        *reinterpret_cast<uint4 *>(row) = re;
+        *reinterpret_cast<uint4 *>(row + kRowBytes) = im;
+    }
+}
+
+template <>
+CORR_HD inline void stage_item<2>(int item, uint8_t *lds, const void *spectra, size_t ys, int nf,
+                                  int frames, int f0, int c0, size_t cpitch) {
+    const int q = item & 3, o = (item >> 2) & 3, s = item >> 4;
+    const uint16_t *y = static_cast<const uint16_t *>(spectra);
+    uint32_t w[16][2];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int f = f0 + 16 * o + i;
+        if (f < frames) {
+            const uint16_t *src = y + (size_t)s * ys + (size_t)f * (size_t)nf + (size_t)(c0 + 4 * q);
+            const uint2 v = *reinterpret_cast<const uint2 *>(src);
+            w[i][0] = v.x;  // channels 0, 1 of the quad: re0 im0 re1 im1
+            w[i][1] = v.y;  // channels 2, 3
+        } else {
+            w[i][0] = w[i][1] = 0u;
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            const int sh = 8 * (2 * (m & 1) + p);  // byte of (channel m, part p)
+            uint4 r;
+            uint32_t *rw = &r.x;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                uint32_t acc = 0;
+#pragma unroll
+                for (int b = 0; b < 4; ++b) acc |= ((w[4 * k + b][m >> 1] >> sh) & 0xFFu) << (8 * b);
+                rw[k] = acc;
+            }
+            uint8_t *row = lds + (size_t)(4 * q + m) * cpitch + (size_t)(2 * s + p) * kRowBytes + 16 * o;
+            *reinterpret_cast<uint4 *>(row) = r;
+        }
+    }
+}
+
+/// Whole X-engine workgroup (16 channels) for `frames` frames; NRT = row tiles.
+/// `ops` provides mma(Frag, Frag, Acc) and xor1(value) as wave-wide operations;
+/// every lane of every wave calls them the same number of times.
+template <int FORM, int NRT, class Sync, class Ops>
+CORR_HD inline void xengine_mfma_block(int tid, int bx, int n_tiles, uint8_t *lds, Sync &sync, Ops &ops,
+                                       const void *spectra, size_t ys, int nf, int frames,
+                                       int n_station, void *integ) {
+    using T = Traits<FORM>;
+    using acc_t = typename T::acc_t;
+    constexpr int NTP = NRT * (NRT + 1) / 2;
+    const int lane = tid & 63, w = tid >> 6;
+    const int c0 = channel_tile(bx, n_tiles) * kChannels;
+    const int rp = NRT * 16;
+    const size_t cpitch = (size_t)rp * kRowBytes + 16;
+    const int n_items = n_station * 16;
+
+    // Rows 2N .. rp-1 are never staged: zero them once.
+    for (int i = tid; i < kChannels * (rp - 2 * n_station) * (kRowBytes / 16); i += kThreads) {
+        const int per_ch = (rp - 2 * n_station) * (kRowBytes / 16);
+        const int ch = i / per_ch, rem = i - ch * per_ch;
+        const int row = 2 * n_station + rem / (kRowBytes / 16), seg = rem % (kRowBytes / 16);
+        *reinterpret_cast<uint4 *>(lds + (size_t)ch * cpitch + (size_t)row * kRowBytes + 16 * seg) =
+            make_uint4(0u, 0u, 0u, 0u);
+    }
+
+    Acc<acc_t> acc[kChanPerWave][NTP];
+#pragma unroll
+    for (int ch = 0; ch < kChanPerWave; ++ch)
+#pragma unroll
+        for (int p = 0; p < NTP; ++p)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[ch][p].v[v] = (acc_t)0;
+
+    for (int f0 = 0; f0 < frames; f0 += T::kFramesPerStep) {
+        sync();
+        for (int item = tid; item < n_items; item += kThreads)
+            stage_item<FORM>(item, lds, spectra, ys, nf, frames, f0, c0, cpitch);
+        sync();
+#pragma unroll
+        for (int ch = 0; ch < kChanPerWave; ++ch) {
+            const uint8_t *base = lds + (size_t)(w * kChanPerWave + ch) * cpitch +
+                                  (size_t)(lane & 15) * kRowBytes + 16 * (lane >> 4);
+            Frag fr[NRT];
+#pragma unroll
+            for (int a = 0; a < NRT; ++a) {
+                const uint4 v = *reinterpret_cast<const uint4 *>(base + (size_t)a * 16 * kRowBytes);
+                fr[a].w[0] = v.x;
+                fr[a].w[1] = v.y;
+                fr[a].w[2] = v.z;
+                fr[a].w[3] = v.w;
+            }
+            int p = 0;
+#pragma unroll
+            for (int a = 0; a < NRT; ++a)
+#pragma unroll
+                for (int b = 0; b <= a; ++b) {
+                    acc[ch][p] = ops.mma(fr[a], fr[b], acc[ch][p]);
+                    ++p;
+                }
+        }
+    }
+
+    // Epilogue: 2x2 blocks of M -> visibilities, into lds[b*16 + channel].
+    sync();
+    acc_t *vout = reinterpret_cast<acc_t *>(lds);
+    const int col = lane & 15, r0 = 4 * (lane >> 4);
+#pragma unroll
+    for (int ch = 0; ch < kChanPerWave; ++ch) {
+        const int cl = w * kChanPerWave + ch;
+        int p = 0;
+#pragma unroll
+        for (int a = 0; a < NRT; ++a)
+#pragma unroll
+            for (int b = 0; b <= a; ++b) {
+#pragma unroll
+                for (int v = 0; v < 4; v += 2) {
+                    const acc_t m_rc = acc[ch][p].v[v];       // M[R][C]
+                    const acc_t m_r1c = acc[ch][p].v[v + 1];  // M[R+1][C]
+                    const acc_t n_rc = ops.xor1(m_rc);        // lane^1: M[R][C^1]
+                    const acc_t n_r1c = ops.xor1(m_r1c);      // lane^1: M[R+1][C^1]
+                    const int R = 16 * a + r0 + v, C = 16 * b + col;
+                    const int k = R >> 1, j = C >> 1;
+                    if ((col & 1) == 0 && k < n_station && j <= k) {
+                        const int bl = j * n_station - (j * (j - 1)) / 2 + (k - j);
+                        acc_t *o = vout + 2 * ((size_t)bl * kChannels + cl);
+                        o[0] = m_rc + n_r1c;  // M[2k][2j] + M[2k+1][2j+1]
+                        o[1] = n_rc - m_r1c;  // M[2k][2j+1] - M[2k+1][2j]
+                    }
+                }
+                ++p;
+            }
+    }
+    sync();
+
+    // Flush into the integrator, channels fastest.
+    const int nb = n_station * (n_station + 1) / 2;
+    for (int i = tid; i < nb * kChannels; i += kThreads) {
+        const int bl = i / kChannels, cl = i - bl * kChannels;
+        const size_t at = (size_t)bl * (size_t)nf + (size_t)(c0 + cl);
+        const acc_t re = vout[2 * i], im = vout[2 * i + 1];
+        if constexpr (FORM == 1) {
+            float4 *g = static_cast<float4 *>(integ) + at;
+            float4 cur = *g;
+            neumaier(cur.x, cur.z, re);
+            neumaier(cur.y, cur.w, im);
+            *g = cur;
+        } else {
+            int2 *g = static_cast<int2 *>(integ) + at;
+            int2 cur = *g;
+            cur.x += re;
+            cur.y += im;
+            *g = cur;
+        }
+    }
+}
+
+}  // namespace xmfma
+}  // namespace corr

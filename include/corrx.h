@@ -53,15 +53,46 @@ corr_status corrx_fengine(void *engine, const int8_t *d_in, size_t station_pitch
                           const uint8_t *d_mask, size_t mask_pitch, int32_t mask_bit0,
                           int32_t frames, void *stream);
 
-/* X-engine: d_acc[b*n_freq + c] (complex64) += sum over the frames of the last
-   fengine call of Y_j conj(Y_k), baseline b = (j,k) in corr_baseline_pairs order. */
+/* X-engine: d_acc[b*n_freq + c] += sum over the frames of the last fengine call
+   of Y_j conj(Y_k), baseline b = (j,k) in corr_baseline_pairs order. The
+   integrator element is the form's (corrx_integrator_bytes):
+     FP32: complex64
+     FP16: 4 floats (sum re, sum im, compensation re, compensation im); the sum
+           of a call is added with Neumaier's compensated summation
+     INT8: 2 int32 (re, im), exact; at most 2^31/98 frames per integration
+   A zeroed integrator starts an integration. */
 corr_status corrx_xengine(void *engine, void *d_acc, int32_t frames, void *stream);
+
+/* Bytes of one integrator element (per baseline and channel) of the engine's form. */
+corr_status corrx_integrator_bytes(void *engine, size_t *bytes);
+
+/* Visibilities of an integration of `frames` frames (the caller's nominal count,
+   as corr_row_end): d_out[b*n_freq + c] complex64, the dump layout of
+   corr_rows_fetch. FP32: d_acc * (1.0f/(float)frames) (d_out may equal d_acc);
+   FP16/INT8: integrator / (g_j[c] g_k[c] frames), evaluated in FP64. */
+corr_status corrx_dump(void *engine, const void *d_acc, uint64_t frames, void *d_out, void *stream);
+
+/* FP16/INT8 forms: set the per-(input, channel) gains from the mean power of
+   `frames` frames (arguments as corrx_fengine, any frame count; missing frames
+   are not counted). FP16 gain = 2^(8-E) for a component RMS m*2^E, m in
+   [0.5,1); INT8 gain = 1/(0.35*RMS). A channel with no power keeps gain 1.
+   Overwrites the spectrum buffer. */
+corr_status corrx_calibrate(void *engine, const int8_t *d_in, size_t station_pitch,
+                            const uint8_t *d_mask, size_t mask_pitch, int32_t mask_bit0,
+                            int32_t frames, void *stream);
+
+/* FP16/INT8 forms: gains [station][n_freq] (float, positive, finite) copied from
+   device memory; and the engine's current gain table (device pointer). */
+corr_status corrx_set_gains(void *engine, const float *d_gains, void *stream);
+corr_status corrx_gains(void *engine, const float **d_gains);
 
 /* d_acc *= 1.0f/(float)frames over n_complex values. */
 corr_status corrx_scale(void *d_acc, size_t n_complex, uint64_t frames, void *stream);
 
 /* The engine's spectrum buffer: [station][chunk_frames][n_freq] elements of the
-   form (complex64 for CORRX_FORM_FP32). */
+   form: complex64 (FP32); two IEEE halves in 32 bits, real in the low half
+   (FP16); two int8 in 16 bits, real in the low byte (INT8). Each is the
+   spectrum times the gain of its (input, channel). */
 corr_status corrx_spectra(void *engine, void **d_spectra, size_t *station_stride_elems);
 
 #ifdef __cplusplus

@@ -14,6 +14,16 @@ struct int2 {
     int x, y;
 };
 inline int2 make_int2(int x, int y) { return int2{x, y}; }
+struct float4 {
+    float x, y, z, w;
+};
+struct uint2 {
+    unsigned x, y;
+};
+struct uint4 {
+    unsigned x, y, z, w;
+};
+inline uint4 make_uint4(unsigned x, unsigned y, unsigned z, unsigned w) { return uint4{x, y, z, w}; }
 
 #ifndef CORR_HD
 #define CORR_HD

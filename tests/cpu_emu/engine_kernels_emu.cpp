@@ -1,6 +1,6 @@
 // The engine's kernel launchers (declared in csrc/engine.hpp) for the CPU build:
-// each launch runs the very kernel bodies of csrc/fft_body.hpp and
-// csrc/xengine_body.hpp over the launch's grid, one host thread per lane of a
+// each launch runs the very kernel bodies of csrc/fft_body.hpp,
+// csrc/xengine_body.hpp and csrc/xmfma_body.hpp over the launch's grid, one host thread per lane of a
 // workgroup and a barrier for the workgroup barrier, with the launch geometry
 // (grid, block, LDS bytes) the device launchers use.
 #include <hip/hip_runtime.h>
@@ -9,9 +9,11 @@
 #include <thread>
 #include <vector>
 
+#include "emu_mfma.hpp"
 #include "engine.hpp"
 #include "fft_body.hpp"
 #include "xengine_body.hpp"
+#include "xmfma_body.hpp"
 
 namespace corr {
 namespace {
@@ -46,6 +48,47 @@ hipError_t fengine_t(const int8_t *in, size_t station_pitch, const int32_t *vali
                                                 station_pitch, valid, valid_stride, tw, wl, out,
                                                 out_station_stride);
              });
+    return hipSuccess;
+}
+
+template <int LOG2N, int FORM>
+hipError_t fengine_form_t(const int8_t *in, size_t station_pitch, const int32_t *valid, size_t valid_stride,
+                          const float2 *tw, const float2 *wl, const float *gain, void *out,
+                          size_t out_station_stride, int frames, int n_station) {
+    constexpr int N = 1 << LOG2N;
+    run_grid(frames, n_station, N / 16, fft::Sizes<N>::kLds * sizeof(float2),
+             [=](int j, int bx, int by, char *lds, HostSync &sync) {
+                 fft::fengine_form_block<LOG2N, FORM>(j, bx, by, reinterpret_cast<float2 *>(lds), sync, in,
+                                                      station_pitch, valid, valid_stride, tw, wl, gain, out,
+                                                      out_station_stride);
+             });
+    return hipSuccess;
+}
+
+template <int FORM, int NRT>
+void xmfma_t(int n_station, const void *spectra, size_t ys, int nf, int frames, void *integ) {
+    const int n_tiles = nf / xmfma::kChannels;
+    EmuMfmaShared *sh = new EmuMfmaShared;
+    run_grid(n_tiles, 1, xmfma::kThreads, xmfma::lds_bytes(n_station),
+             [=](int t, int bx, int, char *lds, HostSync &sync) {
+                 EmuMfmaOps ops{t, sh, &sync};
+                 xmfma::xengine_mfma_block<FORM, NRT>(t, bx, n_tiles, reinterpret_cast<uint8_t *>(lds), sync,
+                                                      ops, spectra, ys, nf, frames, n_station, integ);
+             });
+    delete sh;
+}
+
+template <int FORM>
+hipError_t xmfma_f(int n_station, const void *spectra, size_t ys, int nf, int frames, void *integ) {
+    switch (xmfma::row_tiles(n_station)) {
+    case 1: xmfma_t<FORM, 1>(n_station, spectra, ys, nf, frames, integ); break;
+    case 2: xmfma_t<FORM, 2>(n_station, spectra, ys, nf, frames, integ); break;
+    case 3: xmfma_t<FORM, 3>(n_station, spectra, ys, nf, frames, integ); break;
+    case 4: xmfma_t<FORM, 4>(n_station, spectra, ys, nf, frames, integ); break;
+    case 5: xmfma_t<FORM, 5>(n_station, spectra, ys, nf, frames, integ); break;
+    case 6: xmfma_t<FORM, 6>(n_station, spectra, ys, nf, frames, integ); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipSuccess;
 }
 
@@ -104,6 +147,92 @@ hipError_t launch_xengine_fp32(const float2 *y, size_t y_station_stride, int n_f
     };
     if (npair != 1 && npair != 2) return hipErrorInvalidValue;
     run_grid((n_freq + 63) / 64, groups, nthreads, shm, body);
+    return hipSuccess;
+}
+
+hipError_t launch_fengine_form(int form, int log2_nf, const int8_t *in, size_t station_pitch,
+                               const int32_t *valid, size_t valid_stride, const float2 *tw,
+                               const float2 *wl, const float *gain, void *out,
+                               size_t out_station_stride, int frames, int n_station, hipStream_t) {
+    if (frames <= 0 || n_station <= 0) return hipSuccess;
+    if (form != 1 && form != 2) return hipErrorInvalidValue;
+    switch (log2_nf) {
+#define E(L)                                                                                            \
+    case L:                                                                                             \
+        return form == 1 ? fengine_form_t<L, 1>(in, station_pitch, valid, valid_stride, tw, wl, gain, out, \
+                                                out_station_stride, frames, n_station)                  \
+                         : fengine_form_t<L, 2>(in, station_pitch, valid, valid_stride, tw, wl, gain, out, \
+                                                out_station_stride, frames, n_station);
+        E(9) E(10) E(11) E(12) E(13) E(14)
+#undef E
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_power(int nf, int frames, int n_station, const float2 *spec, size_t spec_station_stride,
+                        const int32_t *valid, size_t valid_stride, float *power, int32_t *count, hipStream_t) {
+    if (frames <= 0 || n_station <= 0 || nf <= 0) return hipSuccess;
+    const int lanes = (nf + 255) / 256 * 256;  // the device grid
+    for (int st = 0; st < n_station; ++st)
+        for (int c = 0; c < lanes; ++c)
+            fft::power_sum(c, st, nf, frames, spec, spec_station_stride, valid, valid_stride, power, count);
+    return hipSuccess;
+}
+
+hipError_t launch_gains(int form, int nf, int n_station, const float *power, const int32_t *count, float *gain,
+                        hipStream_t) {
+    for (int st = 0; st < n_station; ++st)
+        for (int c = 0; c < nf; ++c)
+            gain[(size_t)st * nf + c] = fft::gain_from_power(form, power[(size_t)st * nf + c], count[st]);
+    return hipSuccess;
+}
+
+hipError_t launch_fill(void *p, size_t n_words, uint32_t v, hipStream_t) {
+    uint32_t *w = static_cast<uint32_t *>(p);
+    for (size_t i = 0; i < n_words; ++i) w[i] = v;
+    return hipSuccess;
+}
+
+hipError_t xengine_mfma_prepare(int form, int n_station) {
+    return (form == 1 || form == 2) && xmfma::row_tiles(n_station) <= xmfma::kMaxRowTiles ? hipSuccess
+                                                                                           : hipErrorInvalidValue;
+}
+
+hipError_t launch_xengine_mfma(int form, int n_station, const void *spectra, size_t ys, int nf, int frames,
+                               void *integ, hipStream_t) {
+    if (frames <= 0) return hipSuccess;
+    if (nf % xmfma::kChannels != 0) return hipErrorInvalidValue;
+    if (form == 1) return xmfma_f<1>(n_station, spectra, ys, nf, frames, integ);
+    if (form == 2) return xmfma_f<2>(n_station, spectra, ys, nf, frames, integ);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_dump(int form, const void *integ, int n_baseline, int nf, const int2 *pairs, const float *gain,
+                       uint64_t frames, float2 *out, hipStream_t) {
+    const float inv = 1.0f / (float)frames;
+    for (int b = 0; b < n_baseline; ++b)
+        for (int c = 0; c < nf; ++c) {
+            const size_t at = (size_t)b * nf + c;
+            if (form == 0) {
+                const float2 v = static_cast<const float2 *>(integ)[at];
+                out[at] = make_float2(v.x * inv, v.y * inv);
+                continue;
+            }
+            const double den = (double)gain[(size_t)pairs[b].x * nf + c] * (double)gain[(size_t)pairs[b].y * nf + c] *
+                               (double)frames;
+            double re, im;
+            if (form == 1) {
+                const float4 v = static_cast<const float4 *>(integ)[at];
+                re = ((double)v.x + (double)v.z) / den;
+                im = ((double)v.y + (double)v.w) / den;
+            } else {
+                const int2 v = static_cast<const int2 *>(integ)[at];
+                re = (double)v.x / den;
+                im = (double)v.y / den;
+            }
+            out[at] = make_float2((float)re, (float)im);
+        }
     return hipSuccess;
 }
 

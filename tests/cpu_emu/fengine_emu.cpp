@@ -1,8 +1,11 @@
 // CPU emulation of the F-engine kernel body (csrc/fft_body.hpp): one host
 // thread per lane, std::barrier for the workgroup barrier, AddressSanitizer on
 // every buffer. Checks each transform against the oracle's FP64 FFT.
+#include <immintrin.h>
+
 #include <cmath>
 #include <cstdint>
+#include <cstring>
 #include <cstdio>
 #include <cstdlib>
 #include <random>
@@ -34,6 +37,102 @@ static void run_block(int f, int st, const int8_t *in, size_t pitch, const int32
         });
     for (auto &t : th) t.join();
     delete[] lds;
+}
+
+template <int LOG2N, int FORM>
+static void run_form_block(int f, int st, const int8_t *in, size_t pitch, const int32_t *valid,
+                           size_t vstride, const float2 *tw, const float2 *wl, const float *gain,
+                           void *out, size_t ostride) {
+    constexpr int N = 1 << LOG2N;
+    constexpr int NT = N / 16;
+    float2 *lds = new float2[Sizes<N>::kLds];
+    std::barrier<> bar(NT);
+    std::vector<std::thread> th;
+    th.reserve(NT);
+    for (int j = 0; j < NT; ++j)
+        th.emplace_back([=, &bar] {
+            HostSync sync{&bar};
+            fengine_form_block<LOG2N, FORM>(j, f, st, lds, sync, in, pitch, valid, vstride, tw, wl, gain,
+                                            out, ostride);
+        });
+    for (auto &t : th) t.join();
+    delete[] lds;
+}
+
+// The FP16 and int8 epilogues write exactly pack(fp32 bin, gain) at the same
+// index as the FP32 epilogue (the FP32 result `ref` of the same frame).
+template <int LOG2N>
+static int check_forms(const int8_t *in, size_t pitch, const int32_t *valid, size_t vstride,
+                       const float2 *tw, const float2 *wl, const float2 *ref, size_t rstride, int nst,
+                       int fr) {
+    constexpr int N = 1 << LOG2N;
+    std::mt19937 rng(77 + LOG2N);
+    std::uniform_int_distribution<int> e(-12, 2);
+    std::uniform_real_distribution<float> u(0.01f, 0.2f);
+    float *g16 = new float[(size_t)nst * N], *g8 = new float[(size_t)nst * N];
+    for (size_t i = 0; i < (size_t)nst * N; ++i) {
+        g16[i] = std::ldexp(1.0f, e(rng));
+        g8[i] = u(rng);
+    }
+    const size_t ostride = (size_t)fr * N;
+    uint32_t *o16 = new uint32_t[ostride * nst];
+    uint16_t *o8 = new uint16_t[ostride * nst];
+    for (int st = 0; st < nst; ++st)
+        for (int f = 0; f < fr; ++f) {
+            run_form_block<LOG2N, 1>(f, st, in, pitch, valid, vstride, tw, wl, g16, o16, ostride);
+            run_form_block<LOG2N, 2>(f, st, in, pitch, valid, vstride, tw, wl, g8, o8, ostride);
+        }
+    int bad = 0;
+    for (int st = 0; st < nst; ++st)
+        for (int f = 0; f < fr; ++f)
+            for (int k = 0; k < N; ++k) {
+                const float2 y = ref[st * rstride + (size_t)f * N + k];
+                const size_t at = st * ostride + (size_t)f * N + k;
+                if (o16[at] != corr::forms::pack_fp16(y, g16[(size_t)st * N + k])) ++bad;
+                if (o8[at] != corr::forms::pack_int8(y, g8[(size_t)st * N + k])) ++bad;
+            }
+    std::printf("N=%d forms: %d mismatches\n", N, bad);
+    delete[] g16;
+    delete[] g8;
+    delete[] o16;
+    delete[] o8;
+    return bad;
+}
+
+// The host's float -> half conversion against the CPU's F16C instruction
+// (round to nearest even) over normals, subnormals, overflow and specials.
+__attribute__((target("f16c"))) static uint16_t f16c(float x) {
+    return (uint16_t)_cvtss_sh(x, _MM_FROUND_TO_NEAREST_INT | _MM_FROUND_NO_EXC);
+}
+
+__attribute__((target("f16c"))) static float f16c_back(uint16_t h) { return _cvtsh_ss(h); }
+
+static int check_half() {
+    int bad = 0;
+    std::mt19937 rng(5);
+    std::vector<uint32_t> probes;
+    for (uint32_t e = 90; e <= 150; ++e)  // around the subnormal, normal and overflow limits
+        for (int i = 0; i < 4000; ++i) probes.push_back((e << 23) | (rng() & 0x7FFFFFu));
+    for (uint32_t m = 0; m < (1u << 14); ++m) probes.push_back((112u << 23) | (m << 9));  // ties
+    const uint32_t special[] = {0u, 0x7F800000u, 0x7FC00000u, 0x477FE000u, 0x477FEFFFu,
+                                0x477FF000u, 0x33000000u, 0x33000001u, 0x387FC000u, 0x38800000u};
+    for (uint32_t s : special) probes.push_back(s);
+    for (uint32_t b : probes)
+        for (uint32_t sign : {0u, 0x80000000u}) {
+            float x;
+            const uint32_t bits = b | sign;
+            std::memcpy(&x, &bits, 4);
+            const uint16_t want = f16c(x), got = corr::forms::f32_to_f16_bits(x);
+            const bool nan = (want & 0x7C00u) == 0x7C00u && (want & 0x3FFu) != 0;
+            if (nan ? !((got & 0x7C00u) == 0x7C00u && (got & 0x3FFu) != 0) : want != got) ++bad;
+        }
+    for (uint32_t h = 0; h < 0x10000u; ++h) {
+        const float want = f16c_back((uint16_t)h), got = corr::forms::f16_bits_to_f32((uint16_t)h);
+        if (!(want == got || (want != want && got != got))) ++bad;
+        if (want == got && std::signbit(want) != std::signbit(got)) ++bad;
+    }
+    std::printf("half conversion: %zu probes, %d mismatches\n", probes.size() * 2 + 0x10000, bad);
+    return bad;
 }
 
 template <int LOG2N>
@@ -92,6 +191,7 @@ static int check() {
             if (!(e < 1e-5)) ++bad;
         }
     std::printf("N=%d: worst relative RMS %.3e, %d bad\n", N, worst, bad);
+    if (LOG2N == 9 || LOG2N == 14) bad += check_forms<LOG2N>(in, pitch, valid, FR, tw, wl, out, ostride, NST, FR);
     delete[] in;
     delete[] mask;
     delete[] valid;
@@ -102,7 +202,8 @@ static int check() {
 }
 
 int main() {
-    int bad = check<9>() + check<10>() + check<11>() + check<12>() + check<13>() + check<14>();
+    int bad = check_half();
+    bad += check<9>() + check<10>() + check<11>() + check<12>() + check<13>() + check<14>();
     std::printf(bad ? "FAILED\n" : "OK\n");
     return bad ? 1 : 0;
 }

@@ -41,10 +41,71 @@ def golden(L, N, nf, n_cum, fps, rows):
     return e < 1e-4 and em < 1e-4 and same
 
 
+def ptr(a):
+    return a.ctypes.data
+
+
+def forms(L, form, N, nf, T, chunk, cal_frames):
+    """Engine path of a precision form in the CPU build: calibrate, then
+    F-engine + X-engine per chunk with a missing frame, then the dump; against
+    the FP64 oracle on the engine's own requantized spectra."""
+    x = oracle.gen_int8(31 + form, N, T, 2 * nf, 0.3, max_delay=4)
+    mask_pitch = (T + 7) // 8 + 1
+    mask = np.full((N, mask_pitch), 0xFF, dtype=np.uint8)
+    mask[1, 0] &= ~np.uint8(1 << 3)  # station 1, frame 3 missing
+    valid = np.ones((N, T), dtype=bool)
+    valid[1, 3] = False
+    e = corr.Engine(N, nf, chunk, form, lib=L)
+    pitch = T * 2 * nf
+    e.calibrate(ptr(x), pitch, cal_frames, ptr(mask), mask_pitch, 0)
+    gains = np.ctypeslib.as_array(ctypes.cast(e.gains(), ctypes.POINTER(ctypes.c_float)),
+                                  shape=(N * nf,)).reshape(N, nf).copy()
+    B = N * (N + 1) // 2
+    ib = e.integrator_bytes()
+    integ = np.zeros(B * nf * ib // 4, dtype=np.uint32)
+    sp, stride = e.spectra()
+    eb = 4 if form == corr.FORM_FP16 else 2
+    Yq = np.zeros((N, T, nf), dtype=np.complex64)
+    for f0 in range(0, T, chunk):
+        n = min(chunk, T - f0)
+        e.fengine(ptr(x) + f0 * 2 * nf, pitch, n, ptr(mask), mask_pitch, f0)
+        raw = np.ctypeslib.as_array(ctypes.cast(sp, ctypes.POINTER(ctypes.c_uint8)),
+                                    shape=(N * stride * eb,)).reshape(N, stride * eb)
+        for s_ in range(N):
+            blk = raw[s_, : n * nf * eb]
+            if form == corr.FORM_FP16:
+                h = blk.view(np.float16).astype(np.float32).reshape(n, nf, 2)
+            else:
+                h = blk.view(np.int8).astype(np.float32).reshape(n, nf, 2)
+            Yq[s_, f0:f0 + n] = h[..., 0] + 1j * h[..., 1]
+        e.xengine(ptr(integ), n)
+    out = np.zeros(B * nf, dtype=np.complex64)
+    e.dump(ptr(integ), T, ptr(out))
+    e.close()
+    V = out.reshape(B, nf)
+    ref_q = oracle.x_f64_from_spectra(Yq, gain=gains, frames_div=T)
+    ref = oracle.fx_f64(x, valid=valid)
+    eq = rel_rms(V, ref_q)
+    ok = eq < 1e-6
+    if form == corr.FORM_INT8:
+        acc = integ.view(np.int32).reshape(B, nf, 2)
+        exact = oracle.x_f64_from_spectra(Yq, frames_div=1)
+        ok = ok and np.array_equal(acc[..., 0], exact.real) and np.array_equal(acc[..., 1], exact.imag)
+    missing_zero = not np.any(Yq[1, 3])
+    e64 = rel_rms(V, ref)
+    bound = 1e-3 if form == corr.FORM_FP16 else 0.3
+    ok = ok and missing_zero and e64 < bound and np.all(gains > 0)
+    print(f"form {form} N={N} nf={nf} T={T} chunk={chunk}: vs oracle on engine spectra {eq:.3e}, "
+          f"vs FP64 {e64:.3e}, missing frame zero {missing_zero}", flush=True)
+    return ok
+
+
 def main():
     L = corr.declare(ctypes.CDLL(os.path.join(HERE, "_build", "libcorr_cuda_emu.so")))
     ok = golden(L, 4, 2048, 90, 32, 2)
     ok = golden(L, 5, 4096, 37, 16, 3) and ok
+    ok = forms(L, corr.FORM_FP16, 5, 512, 70, 32, 20) and ok
+    ok = forms(L, corr.FORM_INT8, 6, 512, 70, 32, 70) and ok
     print("OK" if ok else "FAILED")
     return 0 if ok else 1
 

@@ -1,0 +1,133 @@
+// CPU emulation of the MFMA X-engine kernel body (csrc/xmfma_body.hpp) with the
+// kernel's launch geometry: one host thread per lane, std::barrier for the
+// workgroup barrier, the MFMA and lane exchange emulated with the kernel's lane
+// layout (emu_mfma.hpp), AddressSanitizer on every buffer. Checks against a
+// direct accumulation of the same spectra: int8 exactly (int64), FP16 against
+// FP64 within FP32 rounding, integrator start values included.
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <random>
+#include <thread>
+#include <vector>
+
+#include "emu_mfma.hpp"
+
+using namespace corr::xmfma;
+
+template <int FORM, int NRT>
+static void run(const void *y, size_t ys, int nf, int frames, int nst, void *integ) {
+    const int n_tiles = nf / kChannels;
+    const size_t shm = lds_bytes(nst);
+    uint8_t *lds = new uint8_t[shm];  // heap, exact size: ASan sees the bounds
+    EmuMfmaShared *sh = new EmuMfmaShared;
+    std::barrier<> bar(kThreads);
+    std::vector<std::thread> th;
+    for (int t = 0; t < kThreads; ++t)
+        th.emplace_back([=, &bar] {
+            HostSync sync{&bar};
+            EmuMfmaOps ops{t, sh, &sync};
+            for (int bx = 0; bx < n_tiles; ++bx) {
+                xengine_mfma_block<FORM, NRT>(t, bx, n_tiles, lds, sync, ops, y, ys, nf, frames, nst, integ);
+                sync();
+            }
+        });
+    for (auto &t : th) t.join();
+    delete sh;
+    delete[] lds;
+}
+
+template <int FORM>
+static void dispatch(int nrt, const void *y, size_t ys, int nf, int frames, int nst, void *integ) {
+    switch (nrt) {
+    case 1: run<FORM, 1>(y, ys, nf, frames, nst, integ); break;
+    case 2: run<FORM, 2>(y, ys, nf, frames, nst, integ); break;
+    case 3: run<FORM, 3>(y, ys, nf, frames, nst, integ); break;
+    case 4: run<FORM, 4>(y, ys, nf, frames, nst, integ); break;
+    case 5: run<FORM, 5>(y, ys, nf, frames, nst, integ); break;
+    case 6: run<FORM, 6>(y, ys, nf, frames, nst, integ); break;
+    }
+}
+
+static int check_int8(int nst, int nf, int frames, int cap) {
+    const size_t ys = (size_t)cap * nf;
+    std::mt19937 rng(nst * 7 + frames);
+    std::uniform_int_distribution<int> d(-7, 7);
+    std::vector<uint16_t> y(ys * nst);
+    for (auto &v : y) v = (uint16_t)((d(rng) & 0xFF) | ((d(rng) & 0xFF) << 8));
+    const int B = nst * (nst + 1) / 2;
+    std::vector<int2> acc((size_t)B * nf);
+    for (size_t i = 0; i < acc.size(); ++i) acc[i] = make_int2((int)(i % 97) - 40, 3 - (int)(i % 13));
+    const std::vector<int2> start = acc;
+    dispatch<2>(row_tiles(nst), y.data(), ys, nf, frames, nst, acc.data());
+    int bad = 0, b = 0;
+    for (int j = 0; j < nst; ++j)
+        for (int k = j; k < nst; ++k, ++b)
+            for (int c = 0; c < nf; ++c) {
+                int64_t sr = start[(size_t)b * nf + c].x, si = start[(size_t)b * nf + c].y;
+                for (int t = 0; t < frames; ++t) {
+                    const uint16_t p = y[j * ys + (size_t)t * nf + c], q = y[k * ys + (size_t)t * nf + c];
+                    const int xr = (int8_t)(p & 0xFF), xi = (int8_t)(p >> 8);
+                    const int yr = (int8_t)(q & 0xFF), yi = (int8_t)(q >> 8);
+                    sr += xr * yr + xi * yi;
+                    si += xi * yr - xr * yi;
+                }
+                const int2 g = acc[(size_t)b * nf + c];
+                if (g.x != sr || g.y != si) ++bad;
+            }
+    std::printf("int8 N=%d nf=%d frames=%d: %d mismatches\n", nst, nf, frames, bad);
+    return bad;
+}
+
+static int check_fp16(int nst, int nf, int frames, int cap) {
+    const size_t ys = (size_t)cap * nf;
+    std::mt19937 rng(nst * 11 + frames);
+    std::normal_distribution<float> d(0.0f, 200.0f);
+    std::vector<uint32_t> y(ys * nst);
+    for (auto &v : y) v = corr::forms::pack_fp16(make_float2(d(rng), d(rng)), 1.0f);
+    const int B = nst * (nst + 1) / 2;
+    std::vector<float4> acc((size_t)B * nf);
+    for (size_t i = 0; i < acc.size(); ++i) acc[i] = float4{1000.0f, -500.0f, 0.25f, -0.125f};
+    dispatch<1>(row_tiles(nst), y.data(), ys, nf, frames, nst, acc.data());
+    double worst = 0;
+    int bad = 0, b = 0;
+    for (int j = 0; j < nst; ++j)
+        for (int k = j; k < nst; ++k, ++b)
+            for (int c = 0; c < nf; ++c) {
+                double sr = 1000.25, si = -500.125, mag = 1.0;
+                for (int t = 0; t < frames; ++t) {
+                    const uint32_t p = y[j * ys + (size_t)t * nf + c], q = y[k * ys + (size_t)t * nf + c];
+                    const double xr = corr::forms::f16_bits_to_f32((uint16_t)p);
+                    const double xi = corr::forms::f16_bits_to_f32((uint16_t)(p >> 16));
+                    const double yr = corr::forms::f16_bits_to_f32((uint16_t)q);
+                    const double yi = corr::forms::f16_bits_to_f32((uint16_t)(q >> 16));
+                    sr += xr * yr + xi * yi;
+                    si += xi * yr - xr * yi;
+                    mag += std::hypot(xr, xi) * std::hypot(yr, yi);
+                }
+                const float4 g = acc[(size_t)b * nf + c];
+                const double e = std::hypot((double)g.x + g.z - sr, (double)g.y + g.w - si) / mag;
+                if (e > worst) worst = e;
+                if (!(e < 2e-6)) ++bad;
+            }
+    std::printf("fp16 N=%d nf=%d frames=%d: worst %.3e, %d bad\n", nst, nf, frames, worst, bad);
+    return bad;
+}
+
+int main() {
+    int bad = 0;
+    // nf 256: 16 channel tiles (XCD mapping on); nf 48: 3 tiles (identity)
+    bad += check_int8(1, 48, 3, 4);
+    bad += check_int8(4, 256, 70, 72);
+    bad += check_int8(9, 48, 130, 130);
+    bad += check_int8(24, 256, 64, 64);
+    bad += check_int8(30, 48, 20, 20);
+    bad += check_int8(48, 48, 65, 66);
+    bad += check_fp16(1, 48, 1, 2);
+    bad += check_fp16(5, 256, 33, 40);
+    bad += check_fp16(8, 48, 64, 64);
+    bad += check_fp16(24, 256, 40, 40);
+    bad += check_fp16(40, 48, 31, 31);
+    std::printf(bad ? "FAILED\n" : "OK\n");
+    return bad ? 1 : 0;
+}

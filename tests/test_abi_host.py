@@ -138,3 +138,28 @@ def test_null_handles_rejected(corr):
     assert lib.corrx_engine_create(None, ctypes.byref(e)) == corr.ERR_ARG and not e.value
     assert lib.corrx_fengine(None, None, 0, None, 0, 0, 1, None) == corr.ERR_ARG
     assert lib.corrx_scale(None, 1, 1, None) == corr.ERR_ARG
+    n = ctypes.c_size_t()
+    assert lib.corrx_integrator_bytes(None, ctypes.byref(n)) == corr.ERR_ARG
+    assert lib.corrx_dump(None, None, 1, None, None) == corr.ERR_ARG
+    assert lib.corrx_calibrate(None, None, 0, None, 0, 0, 1, None) == corr.ERR_ARG
+    assert lib.corrx_set_gains(None, None, None) == corr.ERR_ARG
+    assert lib.corrx_gains(None, ctypes.byref(ctypes.c_void_p())) == corr.ERR_ARG
+
+
+@pytest.mark.parametrize("shape,status", [
+    ((4, 1024, 16, 7), "ERR_ARG"),        # unknown form
+    ((4, 1024, 16, -1), "ERR_ARG"),
+    ((49, 1024, 16, 1), "ERR_NOT_IMPL"),  # FP16 form: at most 48 inputs
+    ((49, 1024, 16, 2), "ERR_NOT_IMPL"),
+    ((4, 1000, 16, 1), "ERR_NOT_IMPL"),   # FFT length
+    ((0, 1024, 16, 2), "ERR_ARG"),
+    ((4, 1024, 0, 1), "ERR_ARG"),
+])
+def test_engine_shape_rejections(corr, shape, status):
+    """Shapes the engine refuses before touching a device."""
+    lib = corr.load()
+    sh = corr.corrx_shape(*shape)
+    e = ctypes.c_void_p(0x1)
+    assert lib.corrx_engine_create(ctypes.byref(sh), ctypes.byref(e)) == getattr(corr, status)
+    assert not e.value
+    assert lib.corr_last_error()

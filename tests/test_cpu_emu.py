@@ -19,7 +19,7 @@ def built():
     return os.path.join(EMU, "_build")
 
 
-@pytest.mark.parametrize("prog", ["fengine_emu", "xengine_emu"])
+@pytest.mark.parametrize("prog", ["fengine_emu", "xengine_emu", "xmfma_emu"])
 def test_kernel_body_emulation(built, prog):
     r = subprocess.run([os.path.join(built, prog)], capture_output=True, text=True, timeout=600,
                        env=dict(os.environ, ASAN_OPTIONS="detect_leaks=0"))

@@ -1,0 +1,236 @@
+"""Precision forms on the GPU (include/corrx.h, CORRX_FORM_FP16 / CORRX_FORM_INT8).
+
+MFMA X-engine alone, on spectra written straight into the engine's spectrum
+buffer: int8 bit-exact against an int64 sum of the same products, FP16 within
+FP32 rounding of an FP64 sum; the integrator's start values are kept (+=), for
+1..48 inputs (every row-tile count the kernel has), ragged frame counts and both
+channel-tile mappings.
+
+Whole form pipeline (calibrate, F-engine per chunk with a missing frame,
+X-engine, FP64 dump) against the FP64 oracle: on the engine's own requantized
+spectra within 1e-6 (int8 integrator exact), and against unquantized FP64
+visibilities, FP16 within the 1e-3 relative RMS the precision form is held to.
+"""
+
+import importlib
+
+import numpy as np
+import pytest
+
+from conftest import rel_rms
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def hip(corr):
+    return importlib.import_module("21cma_abf_correlator_amd.hip")
+
+
+def _upload_spectra(hip, eng, host_words):
+    """host_words: (N, stride) array of the form's element (uint32 / uint16)."""
+    ptr, stride = eng.spectra()
+    assert host_words.shape[1] == stride
+    a = np.ascontiguousarray(host_words)
+    hip.check(hip.rt().hipMemcpy(ptr, a.ctypes.data, a.nbytes, hip.H2D), "hipMemcpy")
+
+
+def _download_spectra(hip, eng, N, T, nf, form):
+    ptr, stride = eng.spectra()
+    eb = 4 if form == 1 else 2
+    raw = np.empty((N, stride * eb), dtype=np.uint8)
+    hip.synchronize()
+    hip.check(hip.rt().hipMemcpy(raw.ctypes.data, ptr, raw.nbytes, hip.D2H), "hipMemcpy")
+    blk = raw[:, : T * nf * eb]
+    if form == 1:
+        h = blk.view(np.float16).astype(np.float32).reshape(N, T, nf, 2)
+    else:
+        h = blk.view(np.int8).astype(np.float32).reshape(N, T, nf, 2)
+    return (h[..., 0] + 1j * h[..., 1]).astype(np.complex64)
+
+
+def _x_int64(q, N):
+    """q: (N, T, nf, 2) int; integer visibilities (B, nf, 2) in j<=k order."""
+    out = []
+    for j in range(N):
+        for k in range(j, N):
+            xr, xi = q[j, ..., 0].astype(np.int64), q[j, ..., 1].astype(np.int64)
+            yr, yi = q[k, ..., 0].astype(np.int64), q[k, ..., 1].astype(np.int64)
+            out.append(np.stack([(xr * yr + xi * yi).sum(0), (xi * yr - xr * yi).sum(0)], -1))
+    return np.stack(out)
+
+
+@pytest.mark.parametrize("N,nf,T", [(1, 512, 3), (4, 512, 70), (9, 1024, 130), (24, 512, 64),
+                                    (24, 16384, 37), (30, 512, 20), (40, 512, 65), (48, 512, 7)])
+def test_xengine_int8_exact(corr, hip, N, nf, T):
+    print(f"int8 MFMA X-engine N={N} nf={nf} T={T}", flush=True)
+    rng = np.random.default_rng(N * 1000 + T)
+    eng = corr.Engine(N, nf, T, corr.FORM_INT8)
+    _, stride = eng.spectra()
+    q = rng.integers(-7, 8, size=(N, T, nf, 2)).astype(np.int8)
+    q[0, 0, : min(nf, 4)] = [[7, -7], [-7, 7], [7, 7], [-7, -7]][: min(nf, 4)]
+    words = np.zeros((N, stride), dtype=np.uint16)
+    words[:, : T * nf] = q.view(np.uint16).reshape(N, T * nf)
+    _upload_spectra(hip, eng, words)
+    B = N * (N + 1) // 2
+    start = rng.integers(-1000, 1000, size=(B, nf, 2)).astype(np.int32)
+    acc = hip.Buffer.from_array(start)
+    eng.xengine(acc.ptr, T)
+    hip.synchronize()
+    got = acc.to_numpy(np.int32, (B, nf, 2)).astype(np.int64)
+    want = _x_int64(q, N) + start
+    bad = np.argwhere(got != want)
+    if len(bad):
+        b, c, p = bad[0]
+        print(f"  first mismatch baseline {b} channel {c} part {p}: {got[b, c, p]} != {want[b, c, p]}"
+              f" ({len(bad)} of {got.size})")
+    assert len(bad) == 0
+    eng.close()
+
+
+@pytest.mark.parametrize("N,nf,T", [(1, 512, 1), (5, 512, 33), (8, 1024, 64), (24, 512, 96),
+                                    (24, 16384, 40), (40, 512, 31), (48, 512, 50)])
+def test_xengine_fp16(corr, hip, N, nf, T):
+    print(f"FP16 MFMA X-engine N={N} nf={nf} T={T}", flush=True)
+    rng = np.random.default_rng(N * 77 + T)
+    eng = corr.Engine(N, nf, T, corr.FORM_FP16)
+    _, stride = eng.spectra()
+    h = rng.normal(0.0, 200.0, size=(N, T, nf, 2)).astype(np.float16)
+    words = np.zeros((N, stride), dtype=np.uint32)
+    words[:, : T * nf] = h.view(np.uint32).reshape(N, T * nf)
+    _upload_spectra(hip, eng, words)
+    B = N * (N + 1) // 2
+    start = np.zeros((B, nf, 4), dtype=np.float32)
+    start[..., 0], start[..., 1], start[..., 2], start[..., 3] = 1000.0, -500.0, 0.25, -0.125
+    acc = hip.Buffer.from_array(start)
+    eng.xengine(acc.ptr, T)
+    hip.synchronize()
+    got = acc.to_numpy(np.float32, (B, nf, 4)).astype(np.float64)
+    hd = h.astype(np.float64)
+    Y = hd[..., 0] + 1j * hd[..., 1]
+    worst = 0.0
+    b = 0
+    for j in range(N):
+        for k in range(j, N):
+            v = (Y[j] * np.conj(Y[k])).sum(0) + (1000.25 - 500.125j)
+            mag = (np.abs(Y[j]) * np.abs(Y[k])).sum(0) + 1.0
+            g = (got[b, :, 0] + got[b, :, 2]) + 1j * (got[b, :, 1] + got[b, :, 3])
+            worst = max(worst, float(np.max(np.abs(g - v) / mag)))
+            b += 1
+    print(f"  worst |error| / sum |products| = {worst:.3e}")
+    assert worst < 2e-6
+    eng.close()
+
+
+def _pipeline(corr, oracle, hip, form, N, nf, T, chunk, cal_frames, rho, seed, max_delay):
+    x = oracle.gen_int8(seed, N, T, 2 * nf, rho, sigma=16.0, max_delay=max_delay)
+    mask_pitch = (T + 7) // 8
+    mask = np.full((N, mask_pitch), 0xFF, dtype=np.uint8)
+    mask[1, 0] &= ~np.uint8(1 << 3)  # station 1, frame 3 missing
+    valid = np.ones((N, T), dtype=np.uint8)
+    valid[1, 3] = 0
+    eng = corr.Engine(N, nf, chunk, form)
+    d_in = hip.Buffer.from_array(x)
+    d_mask = hip.Buffer.from_array(mask)
+    pitch = T * 2 * nf
+    eng.calibrate(d_in.ptr, pitch, cal_frames, d_mask.ptr, mask_pitch, 0)
+    hip.synchronize()
+    gains = np.empty((N, nf), dtype=np.float32)
+    hip.check(hip.rt().hipMemcpy(gains.ctypes.data, eng.gains(), gains.nbytes, hip.D2H), "hipMemcpy")
+    B = N * (N + 1) // 2
+    acc = hip.Buffer(B * nf * eng.integrator_bytes())
+    acc.zero()
+    Yq = np.zeros((N, T, nf), dtype=np.complex64)
+    for f0 in range(0, T, chunk):
+        n = min(chunk, T - f0)
+        eng.fengine(d_in.ptr + f0 * 2 * nf, pitch, n, d_mask.ptr, mask_pitch, f0)
+        Yq[:, f0:f0 + n] = _download_spectra(hip, eng, N, n, nf, form)
+        eng.xengine(acc.ptr, n)
+    out = hip.Buffer(B * nf * 8)
+    eng.dump(acc.ptr, T, out.ptr)
+    hip.synchronize()
+    V = out.to_numpy(np.complex64, (B, nf))
+    if form == corr.FORM_INT8:
+        raw = acc.to_numpy(np.int32, (B, nf, 2))
+    else:
+        raw = acc.to_numpy(np.float32, (B, nf, 4))
+    eng.close()
+    return x, valid, gains, Yq, V, raw
+
+
+@pytest.mark.parametrize("form", [1, 2])
+def test_form_pipeline(corr, oracle, hip, form):
+    N, nf, T, chunk = 6, 2048, 100, 32
+    print(f"form {form} pipeline N={N} nf={nf} T={T} chunk={chunk}", flush=True)
+    x, valid, gains, Yq, V, raw = _pipeline(corr, oracle, hip, form, N, nf, T, chunk, 60, 0.3, 5, 8)
+    assert np.all(Yq[1, 3] == 0)
+    assert np.all(gains > 0) and np.all(np.isfinite(gains))
+    ref_q = oracle.x_f64_from_spectra(Yq, gain=gains, frames_div=T)
+    eq = rel_rms(V, ref_q)
+    ref = oracle.fx_f64(x, valid=valid)
+    e64 = rel_rms(V, ref)
+    print(f"  vs oracle on the engine's spectra {eq:.3e}; vs unquantized FP64 {e64:.3e}")
+    assert eq < 1e-6
+    if form == corr.FORM_INT8:
+        exact = oracle.x_f64_from_spectra(Yq, frames_div=1)
+        assert np.array_equal(raw[..., 0], exact.real) and np.array_equal(raw[..., 1], exact.imag)
+        assert e64 < 0.1
+    else:
+        assert e64 < 1e-3
+
+
+@pytest.mark.parametrize("form", [1, 2])
+def test_calibrated_gains(corr, oracle, hip, form):
+    """Gains against the rule applied to the oracle's FP64 spectra of the same
+    frames: FP16 gains are powers of two (equal, or one octave off where the
+    RMS sits on a power-of-two boundary), int8 gains within 1e-4."""
+    N, nf, T = 4, 1024, 40
+    x = oracle.gen_int8(9, N, T, 2 * nf, 0.2, sigma=16.0, max_delay=4, tone_channel=300, tone_amp=8.0)
+    eng = corr.Engine(N, nf, 16, form)
+    d_in = hip.Buffer.from_array(x)
+    eng.calibrate(d_in.ptr, T * 2 * nf, T)
+    hip.synchronize()
+    g = np.empty((N, nf), dtype=np.float32)
+    hip.check(hip.rt().hipMemcpy(g.ctypes.data, eng.gains(), g.nbytes, hip.D2H), "hipMemcpy")
+    eng.close()
+    Y = oracle.spectra_f64(x)
+    sigma = np.sqrt((np.abs(Y) ** 2).mean(axis=1) / 2.0)
+    if form == corr.FORM_FP16:
+        m, e = np.frexp(sigma)
+        want = np.ldexp(1.0, 8 - e)
+        ratio = g / want
+        assert np.all(np.isin(ratio, [0.5, 1.0, 2.0]))
+        assert np.mean(ratio == 1.0) > 0.99
+        assert np.all(np.log2(g) == np.round(np.log2(g)))
+    else:
+        want = 1.0 / (0.35 * sigma)
+        assert np.max(np.abs(g / want - 1.0)) < 1e-4
+
+
+def test_c3_shape_fp16_error(corr, oracle, hip):
+    """The precision-form geometry (24 inputs, n_freq 16384, rho 0.1, delays in
+    [0, 32]) on 64 spectra: FP16 MFMA form within 1e-3 of FP64; the FP32 form's
+    error reported beside it."""
+    N, nf, T = 24, 16384, 64
+    print(f"C3 shape FP16 vs FP64 N={N} nf={nf} T={T}", flush=True)
+    x, valid, gains, Yq, V, raw = _pipeline(corr, oracle, hip, 1, N, nf, T, 32, 64, 0.1, 3, 32)
+    ref = oracle.fx_f64(x, valid=valid)
+    e16 = rel_rms(V, ref)
+    B = N * (N + 1) // 2
+    eng = corr.Engine(N, nf, 32)
+    d_in = hip.Buffer.from_array(x)
+    mask = np.full((N, T // 8), 0xFF, dtype=np.uint8)
+    mask[1, 0] &= ~np.uint8(1 << 3)
+    d_mask = hip.Buffer.from_array(mask)
+    acc = hip.Buffer(B * nf * 8)
+    acc.zero()
+    for f0 in range(0, T, 32):
+        eng.fengine(d_in.ptr + f0 * 2 * nf, T * 2 * nf, 32, d_mask.ptr, T // 8, f0)
+        eng.xengine(acc.ptr, 32)
+    eng.dump(acc.ptr, T, acc.ptr)
+    hip.synchronize()
+    e32 = rel_rms(acc.to_numpy(np.complex64, (B, nf)), ref)
+    eng.close()
+    print(f"  relative RMS vs FP64: FP16 form {e16:.3e}, FP32 form {e32:.3e}")
+    assert e16 < 1e-3
+    assert e32 < 1e-5
