@@ -20,12 +20,24 @@
 
 #include "forms.hpp"
 
+// Every function of the F-engine body is inlined into its kernel: an outlined
+// call would pass the LDS array as a generic pointer (address-space casts on
+// every access) and pay the call's register save and restore. The CPU build
+// leaves inlining to the compiler.
+#ifndef CORR_FI
+#if defined(__HIP_DEVICE_COMPILE__)
+#define CORR_FI __attribute__((always_inline)) inline
+#else
+#define CORR_FI inline
+#endif
+#endif
+
 namespace corr {
 namespace fft {
 
-CORR_HD inline float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
-CORR_HD inline float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
-CORR_HD inline float2 cmul(float2 a, float2 w) {
+CORR_HD CORR_FI float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+CORR_HD CORR_FI float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+CORR_HD CORR_FI float2 cmul(float2 a, float2 w) {
     return make_float2(a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x);
 }
 
@@ -48,7 +60,7 @@ CORR_HD constexpr float cos32(int m) {
 
 // x * exp(-2 pi i M / 32) for a compile-time M in [0, 16).
 template <int M>
-CORR_HD inline float2 rot32(float2 x) {
+CORR_HD CORR_FI float2 rot32(float2 x) {
     if constexpr (M == 0) {
         return x;
     } else if constexpr (M == 8) {  // -i
@@ -68,7 +80,7 @@ CORR_HD inline float2 rot32(float2 x) {
 
 // One radix-2 DIF stage of an in-register DFT of size R: blocks of 2*H.
 template <int R, int H, int BASE, int K>
-CORR_HD inline void dif_butterflies(float2 *v) {
+CORR_HD CORR_FI void dif_butterflies(float2 *v) {
     if constexpr (BASE < R) {
         if constexpr (K < H) {
             const float2 a = v[BASE + K], b = v[BASE + K + H];
@@ -82,7 +94,7 @@ CORR_HD inline void dif_butterflies(float2 *v) {
 }
 
 template <int R, int H>
-CORR_HD inline void dif_stages(float2 *v) {
+CORR_HD CORR_FI void dif_stages(float2 *v) {
     if constexpr (H >= 1) {
         dif_butterflies<R, H, 0, 0>(v);
         dif_stages<R, H / 2>(v);
@@ -100,7 +112,7 @@ CORR_HD constexpr int bitrev(int i) {
 }
 
 template <int R, int I>
-CORR_HD inline void unscramble(float2 *v) {
+CORR_HD CORR_FI void unscramble(float2 *v) {
     if constexpr (I < R) {
         constexpr int J = bitrev<R>(I);
         if constexpr (J > I) {
@@ -115,14 +127,14 @@ CORR_HD inline void unscramble(float2 *v) {
 // In-place DFT of R (power of two, <= 32) values, natural order in and out:
 // v[k] := sum_r v[r] exp(-2 pi i r k / R).
 template <int R>
-CORR_HD inline void dft(float2 *v) {
+CORR_HD CORR_FI void dft(float2 *v) {
     dif_stages<R, R / 2>(v);
     unscramble<R, 0>(v);
 }
 
 // LDS index padding: one spare float2 every 16, so that a wave writing with a
 // stride of 16 elements (the first Stockham pass) spreads over all banks.
-CORR_HD inline int pad16(int a) { return a + (a >> 4); }
+CORR_HD CORR_FI int pad16(int a) { return a + (a >> 4); }
 
 template <int N>
 struct Sizes {
@@ -132,13 +144,21 @@ struct Sizes {
 
 // One Stockham pass of radix R over N points, NS = points already combined,
 // in place in `lds` (barrier between all reads and any write). Thread j does
-// 16/R butterflies jj = j + q*N/16.
+// 16/R butterflies jj = j + q*N/16. The pass's twiddles are requested before
+// its LDS reads and barrier, so their cache latency overlaps the exchange.
 template <int N, int R, int NS, class Sync>
-CORR_HD inline void stockham_pass(float2 (&v)[16], float2 *lds, const float2 *tw, int j, Sync &sync) {
+CORR_HD CORR_FI void stockham_pass(float2 (&v)[16], float2 *lds, const float2 *tw, int j, Sync &sync) {
     constexpr int NT = N / 16;
     constexpr int NB = 16 / R;
     constexpr int STRIDE = N / R;
     constexpr int TWSTEP = N / (NS * R);
+    float2 w[NB][R - 1];
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+        const int k = (j + q * NT) & (NS - 1);
+#pragma unroll
+        for (int r = 1; r < R; ++r) w[q][r - 1] = tw[(unsigned)(r * k * TWSTEP)];
+    }
 #pragma unroll
     for (int q = 0; q < NB; ++q) {
         const int jj = j + q * NT;
@@ -151,7 +171,7 @@ CORR_HD inline void stockham_pass(float2 (&v)[16], float2 *lds, const float2 *tw
         const int jj = j + q * NT;
         const int k = jj & (NS - 1);
 #pragma unroll
-        for (int r = 1; r < R; ++r) v[q * R + r] = cmul(v[q * R + r], tw[r * k * TWSTEP]);
+        for (int r = 1; r < R; ++r) v[q * R + r] = cmul(v[q * R + r], w[q][r - 1]);
         dft<R>(&v[q * R]);
         const int base = (jj - k) * R + k;
 #pragma unroll
@@ -161,7 +181,7 @@ CORR_HD inline void stockham_pass(float2 (&v)[16], float2 *lds, const float2 *tw
 }
 
 template <int LOG2N, int DONE, class Sync>
-CORR_HD inline void stockham_rest(float2 (&v)[16], float2 *lds, const float2 *tw, int j, Sync &sync) {
+CORR_HD CORR_FI void stockham_rest(float2 (&v)[16], float2 *lds, const float2 *tw, int j, Sync &sync) {
     if constexpr (DONE < LOG2N) {
         constexpr int REM = LOG2N - DONE;
         constexpr int LOGR = REM >= 4 ? 4 : REM;
@@ -172,44 +192,47 @@ CORR_HD inline void stockham_rest(float2 (&v)[16], float2 *lds, const float2 *tw
 
 // Validity bit of frame f of station st: bit (bit0 + f) of mask + st*mask_pitch,
 // LSB first within a byte (reference kernels.cu:163-166).
-CORR_HD inline int32_t mask_bit(const uint8_t *mask, size_t mask_pitch, int bit0, int f, int st) {
+CORR_HD CORR_FI int32_t mask_bit(const uint8_t *mask, size_t mask_pitch, int bit0, int f, int st) {
     const int bit = bit0 + f;
     return (int32_t)((mask[(size_t)st * mask_pitch + (size_t)(bit >> 3)] >> (bit & 7)) & 1u);
 }
 
 // Expand the validity bits of `frames` frames into one int32 per (station,
 // frame): valid[st*valid_stride + f]. Thread f of the workgroup of station st.
-CORR_HD inline void expand_validity(int f, int st, int frames, const uint8_t *mask, size_t mask_pitch,
+CORR_HD CORR_FI void expand_validity(int f, int st, int frames, const uint8_t *mask, size_t mask_pitch,
                                     int bit0, int32_t *valid, size_t valid_stride) {
     if (f < frames) valid[(size_t)st * valid_stride + (size_t)f] = mask_bit(mask, mask_pitch, bit0, f, st);
 }
 
-// The Z spectrum of frame f of station st into `lds` (natural order, padded).
-// Thread j of N/16; on return `lds` holds Z after a final barrier.
-// `valid` is NULL (every frame present) or the expanded validity of
-// expand_validity(): one aligned int32 per (station, frame), read once per
-// workgroup at a uniform address.
-template <int LOG2N, class Sync>
-CORR_HD inline void complex_fft_block(int j, int f, int st, float2 *lds, Sync &sync, const int8_t *in,
-                                      size_t station_pitch, const int32_t *valid, size_t valid_stride,
-                                      const float2 *tw) {
-    constexpr int N = 1 << LOG2N;
-    constexpr int NT = N / 16;
-    float2 v[16];
-    if (valid == nullptr || valid[(size_t)st * valid_stride + (size_t)f] != 0) {
+// Thread j's 16 input values of frame f of station st: z[j + r*N/16], r < 16,
+// as int8 pairs; zeros when the frame is missing (it is then never read, which
+// is bit for bit what a frame the host had zeroed gives).
+template <int LOG2N>
+CORR_HD CORR_FI void load_frame(int j, const int8_t *in, size_t station_pitch, int f, int st, bool present,
+                                char2 (&raw)[16]) {
+    constexpr int NT = (1 << LOG2N) / 16;
+    if (present) {
         const char2 *src = reinterpret_cast<const char2 *>(in + (size_t)st * station_pitch +
-                                                           (size_t)f * (size_t)(2 * N));
+                                                           (size_t)f * (size_t)(2 << LOG2N));
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const char2 p = src[j + r * NT];
-            v[r] = make_float2((float)p.x, (float)p.y);
-        }
+        for (int r = 0; r < 16; ++r) raw[r] = src[(unsigned)(j + r * NT)];
     } else {
-        // A missing frame is never read: zeros are transformed instead, which is
-        // bit for bit what a frame the host had zeroed gives.
 #pragma unroll
-        for (int r = 0; r < 16; ++r) v[r] = make_float2(0.0f, 0.0f);
+        for (int r = 0; r < 16; ++r) raw[r].x = raw[r].y = 0;
     }
+}
+
+// Frame f of station st is present: `valid` is NULL (every frame present) or
+// the expanded validity of expand_validity(), one aligned int32 per (station,
+// frame), read once per workgroup at a uniform address.
+CORR_HD CORR_FI bool frame_present(const int32_t *valid, size_t valid_stride, int f, int st) {
+    return valid == nullptr || valid[(size_t)st * valid_stride + (size_t)f] != 0;
+}
+
+// The Z spectrum of thread j's converted input values into `lds` (natural
+// order, padded); on return `lds` holds Z after a final barrier.
+template <int LOG2N, class Sync>
+CORR_HD CORR_FI void fft_from_values(int j, float2 (&v)[16], float2 *lds, Sync &sync, const float2 *tw) {
     // first pass: radix 16, NS = 1, inputs z[j + r*N/16] already in registers
     dft<16>(v);
 #pragma unroll
@@ -218,68 +241,105 @@ CORR_HD inline void complex_fft_block(int j, int f, int st, float2 *lds, Sync &s
     stockham_rest<LOG2N, 4>(v, lds, tw, j, sync);
 }
 
+// The Z spectrum of frame f of station st into `lds` (natural order, padded).
+// Thread j of N/16; on return `lds` holds Z after a final barrier.
+template <int LOG2N, class Sync>
+CORR_HD CORR_FI void complex_fft_block(int j, int f, int st, float2 *lds, Sync &sync, const int8_t *in,
+                                      size_t station_pitch, const int32_t *valid, size_t valid_stride,
+                                      const float2 *tw) {
+    char2 raw[16];
+    load_frame<LOG2N>(j, in, station_pitch, f, st, frame_present(valid, valid_stride, f, st), raw);
+    float2 v[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = make_float2((float)raw[r].x, (float)raw[r].y);
+    fft_from_values<LOG2N>(j, v, lds, sync, tw);
+}
+
 // R2C bin k from Z in lds: (Z[k] + conj(Z[N-k]))/2 + W^k (Z[k] - conj(Z[N-k]))/(2i).
 template <int N>
-CORR_HD inline float2 r2c_bin(const float2 *lds, const float2 *wl, int k) {
+CORR_HD CORR_FI float2 r2c_bin(const float2 *lds, const float2 *wl, int k) {
     const float2 a = lds[pad16(k)];
     const float2 b = lds[pad16((N - k) & (N - 1))];
     const float er = 0.5f * (a.x + b.x), ei = 0.5f * (a.y - b.y);
     const float orr = 0.5f * (a.y + b.y), oi = -0.5f * (a.x - b.x);
-    const float2 w = wl[k];
+    const float2 w = wl[(unsigned)k];
     return make_float2(er + (w.x * orr - w.y * oi), ei + (w.x * oi + w.y * orr));
+}
+
+// Bins k and N-k together (1 <= k < N): with E = (Z[k] + conj(Z[N-k]))/2 and
+// O = (Z[k] - conj(Z[N-k]))/(2i), X[k] = E + W^k O and, as W^(N-k) = -conj(W^k),
+// X[N-k] = conj(E - W^k O).
+template <int N>
+CORR_HD CORR_FI void r2c_pair(const float2 *lds, const float2 *wl, int k, float2 &xk, float2 &xm) {
+    const float2 a = lds[pad16(k)];
+    const float2 b = lds[pad16(N - k)];
+    const float er = 0.5f * (a.x + b.x), ei = 0.5f * (a.y - b.y);
+    const float orr = 0.5f * (a.y + b.y), oi = -0.5f * (a.x - b.x);
+    const float2 w = wl[(unsigned)k];
+    const float tr = w.x * orr - w.y * oi, ti = w.x * oi + w.y * orr;
+    xk = make_float2(er + tr, ei + ti);
+    xm = make_float2(er - tr, ti - ei);
+}
+
+// The R2C epilogue of one workgroup: every retained bin k = 0..N-1 exactly once,
+// thread j taking k = j + q*N/16 (q < 8) with its mirror N - k, thread 0 also
+// the self-mirrored bin N/2 (bin 0's mirror is the Nyquist bin, never formed).
+// emit(k, X) stores one bin.
+template <int N, class Emit>
+CORR_HD CORR_FI void r2c_epilogue(int j, const float2 *lds, const float2 *wl, Emit &&emit) {
+    constexpr int NT = N / 16;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int k = j + q * NT;
+        if (k == 0) {
+            emit(0, r2c_bin<N>(lds, wl, 0));
+        } else {
+            float2 xk, xm;
+            r2c_pair<N>(lds, wl, k, xk, xm);
+            emit(k, xk);
+            emit(N - k, xm);
+        }
+    }
+    if (j == 0) emit(N / 2, r2c_bin<N>(lds, wl, N / 2));
 }
 
 // Whole F-engine workgroup, complex64 output [station][frame][N].
 template <int LOG2N, class Sync>
-CORR_HD inline void fengine_fp32_block(int j, int f, int st, float2 *lds, Sync &sync, const int8_t *in,
+CORR_HD CORR_FI void fengine_fp32_block(int j, int f, int st, float2 *lds, Sync &sync, const int8_t *in,
                                        size_t station_pitch, const int32_t *valid, size_t valid_stride,
                                        const float2 *tw, const float2 *wl, float2 *out,
                                        size_t out_station_stride) {
     constexpr int N = 1 << LOG2N;
-    constexpr int NT = N / 16;
     complex_fft_block<LOG2N>(j, f, st, lds, sync, in, station_pitch, valid, valid_stride, tw);
     float2 *dst = out + (size_t)st * out_station_stride + (size_t)f * (size_t)N;
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int k = j + q * NT;
-        dst[k] = r2c_bin<N>(lds, wl, k);
-    }
+    r2c_epilogue<N>(j, lds, wl, [&](int k, float2 x) { dst[(unsigned)k] = x; });
 }
 
 // Whole F-engine workgroup for the FP16 (FORM 1) or int8 (FORM 2) form: bin k
 // of station st is scaled by gain[st*N + k] and converted (forms.hpp); output
 // [station][frame][N] of packed 32-bit (FP16) or 16-bit (int8) complex.
 template <int LOG2N, int FORM, class Sync>
-CORR_HD inline void fengine_form_block(int j, int f, int st, float2 *lds, Sync &sync, const int8_t *in,
+CORR_HD CORR_FI void fengine_form_block(int j, int f, int st, float2 *lds, Sync &sync, const int8_t *in,
                                        size_t station_pitch, const int32_t *valid, size_t valid_stride,
                                        const float2 *tw, const float2 *wl, const float *gain,
                                        void *out, size_t out_station_stride) {
     constexpr int N = 1 << LOG2N;
-    constexpr int NT = N / 16;
     complex_fft_block<LOG2N>(j, f, st, lds, sync, in, station_pitch, valid, valid_stride, tw);
     const float *g = gain + (size_t)st * N;
     const size_t at = (size_t)st * out_station_stride + (size_t)f * (size_t)N;
     if constexpr (FORM == 1) {
         uint32_t *dst = static_cast<uint32_t *>(out) + at;
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int k = j + q * NT;
-            dst[k] = forms::pack_fp16(r2c_bin<N>(lds, wl, k), g[k]);
-        }
+        r2c_epilogue<N>(j, lds, wl, [&](int k, float2 x) { dst[(unsigned)k] = forms::pack_fp16(x, g[(unsigned)k]); });
     } else {
         uint16_t *dst = static_cast<uint16_t *>(out) + at;
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int k = j + q * NT;
-            dst[k] = forms::pack_int8(r2c_bin<N>(lds, wl, k), g[k]);
-        }
+        r2c_epilogue<N>(j, lds, wl, [&](int k, float2 x) { dst[(unsigned)k] = forms::pack_int8(x, g[(unsigned)k]); });
     }
 }
 
 // Calibration: power[st*nf + c] += sum over the frames present of |Y|^2, in
 // frame order; count[st] += frames present (by the channel-0 thread).
 // Thread c of the workgroup row of station st.
-CORR_HD inline void power_sum(int c, int st, int nf, int frames, const float2 *spec, size_t spec_station_stride,
+CORR_HD CORR_FI void power_sum(int c, int st, int nf, int frames, const float2 *spec, size_t spec_station_stride,
                               const int32_t *valid, size_t valid_stride, float *power, int32_t *count) {
     if (c >= nf) return;
     const float2 *y = spec + (size_t)st * spec_station_stride + (size_t)c;
@@ -299,7 +359,7 @@ CORR_HD inline void power_sum(int c, int st, int nf, int frames, const float2 *s
 //   FP16: 2^(8-E) with sigma = m 2^E, m in [0.5, 1): the component RMS lands in [128, 256)
 //   int8: 1/(0.35 sigma)
 // sigma = per-component RMS = sqrt(mean |Y|^2 / 2); gain 1 where nothing was seen.
-CORR_HD inline float gain_from_power(int form, float power, int32_t count) {
+CORR_HD CORR_FI float gain_from_power(int form, float power, int32_t count) {
     if (count <= 0 || !(power > 0.0f) || !(power < 3.0e38f)) return 1.0f;
     const float sigma = sqrtf(power / (float)count * 0.5f);
     if (!(sigma > 0.0f)) return 1.0f;

@@ -78,11 +78,15 @@ CORR_HD inline int rows_padded(int n_station) { return ((2 * n_station + 15) / 1
 CORR_HD inline int row_tiles(int n_station) { return rows_padded(n_station) / 16; }
 CORR_HD inline size_t chan_pitch(int n_station) { return (size_t)rows_padded(n_station) * kRowBytes + 16; }
 
-/// Dynamic LDS bytes: the staging tile, reused for the [baseline][16] results.
+/// Results of one call in LDS: [baseline][kOutPitch] complex, the pitch one
+/// more than the channels so that one channel's baselines spread over banks.
+constexpr int kOutPitch = kChannels + 1;
+
+/// Dynamic LDS bytes: the staging tile, reused for the results.
 CORR_HD inline size_t lds_bytes(int n_station) {
     const size_t stage = (size_t)kChannels * chan_pitch(n_station);
     const size_t nb = (size_t)n_station * (size_t)(n_station + 1) / 2;
-    const size_t out = nb * kChannels * 8;
+    const size_t out = nb * kOutPitch * 8;
     return stage > out ? stage : out;
 }
 
@@ -250,7 +254,7 @@ CORR_HD inline void xengine_mfma_block(int tid, int bx, int n_tiles, uint8_t *ld
         }
     }
 
-    // Epilogue: 2x2 blocks of M -> visibilities, into lds[b*16 + channel].
+    // Epilogue: 2x2 blocks of M -> visibilities, into lds[b*kOutPitch + channel].
     sync();
     acc_t *vout = reinterpret_cast<acc_t *>(lds);
     const int col = lane & 15, r0 = 4 * (lane >> 4);
@@ -272,7 +276,7 @@ CORR_HD inline void xengine_mfma_block(int tid, int bx, int n_tiles, uint8_t *ld
                     const int k = R >> 1, j = C >> 1;
                     if ((col & 1) == 0 && k < n_station && j <= k) {
                         const int bl = j * n_station - (j * (j - 1)) / 2 + (k - j);
-                        acc_t *o = vout + 2 * ((size_t)bl * kChannels + cl);
+                        acc_t *o = vout + 2 * ((size_t)bl * kOutPitch + cl);
                         o[0] = m_rc + n_r1c;  // M[2k][2j] + M[2k+1][2j+1]
                         o[1] = n_rc - m_r1c;  // M[2k][2j+1] - M[2k+1][2j]
                     }
@@ -287,7 +291,8 @@ CORR_HD inline void xengine_mfma_block(int tid, int bx, int n_tiles, uint8_t *ld
     for (int i = tid; i < nb * kChannels; i += kThreads) {
         const int bl = i / kChannels, cl = i - bl * kChannels;
         const size_t at = (size_t)bl * (size_t)nf + (size_t)(c0 + cl);
-        const acc_t re = vout[2 * i], im = vout[2 * i + 1];
+        const size_t o = 2 * ((size_t)bl * kOutPitch + cl);
+        const acc_t re = vout[o], im = vout[o + 1];
         if constexpr (FORM == 1) {
             float4 *g = static_cast<float4 *>(integ) + at;
             float4 cur = *g;
