@@ -142,6 +142,31 @@ struct Sizes {
     static constexpr int kLds = N + N / 16;  // float2 elements
 };
 
+// Largest power of two <= r (r >= 1), at compile time.
+CORR_HD constexpr int high_bit(int r) { return r >= 16 ? 16 : r >= 8 ? 8 : r >= 4 ? 4 : r >= 2 ? 2 : 1; }
+
+// The twiddles w^r, r = 1..R-1, of one butterfly group, w = exp(-2 pi i m/N)
+// with m = k*TWSTEP: the powers of two w, w^2, w^4, w^8 are read from the
+// FP64-built table, every other power is the product of two earlier ones
+// (w^r = w^(2^h) w^(r - 2^h), at most four rounded factors). Table reads are
+// the F-engine's costliest memory operation (a wave's lanes read scattered
+// entries), so this trades eleven of them per radix-16 group for complex products.
+template <int R, int TWSTEP>
+CORR_HD CORR_FI void group_twiddles(const float2 *tw, int k, float2 (&w)[R - 1]) {
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+        if ((r & (r - 1)) == 0)
+            w[r - 1] = tw[(unsigned)(r * k * TWSTEP)];
+    }
+#pragma unroll
+    for (int r = 3; r < R; ++r) {
+        if ((r & (r - 1)) != 0) {
+            const int h = high_bit(r);
+            w[r - 1] = cmul(w[h - 1], w[r - h - 1]);
+        }
+    }
+}
+
 // One Stockham pass of radix R over N points, NS = points already combined,
 // in place in `lds` (barrier between all reads and any write). Thread j does
 // 16/R butterflies jj = j + q*N/16. The pass's twiddles are requested before
@@ -153,11 +178,20 @@ CORR_HD CORR_FI void stockham_pass(float2 (&v)[16], float2 *lds, const float2 *t
     constexpr int STRIDE = N / R;
     constexpr int TWSTEP = N / (NS * R);
     float2 w[NB][R - 1];
+    if constexpr (NS * R == N && NB > 1) {
+        // Last pass: k = j + q*N/16 exactly, so group q's twiddles are group 0's
+        // times exp(-2 pi i r q/16), constants.
+        group_twiddles<R, TWSTEP>(tw, j, w[0]);
 #pragma unroll
-    for (int q = 0; q < NB; ++q) {
-        const int k = (j + q * NT) & (NS - 1);
+        for (int q = 1; q < NB; ++q)
 #pragma unroll
-        for (int r = 1; r < R; ++r) w[q][r - 1] = tw[(unsigned)(r * k * TWSTEP)];
+            for (int r = 1; r < R; ++r) {
+                const int m = (2 * r * q) & 31;  // exp(-2 pi i m/32)
+                w[q][r - 1] = cmul(w[0][r - 1], make_float2(cos32(m), -cos32((m + 24) & 31)));
+            }
+    } else {
+#pragma unroll
+        for (int q = 0; q < NB; ++q) group_twiddles<R, TWSTEP>(tw, (j + q * NT) & (NS - 1), w[q]);
     }
 #pragma unroll
     for (int q = 0; q < NB; ++q) {
@@ -270,12 +304,11 @@ CORR_HD CORR_FI float2 r2c_bin(const float2 *lds, const float2 *wl, int k) {
 // O = (Z[k] - conj(Z[N-k]))/(2i), X[k] = E + W^k O and, as W^(N-k) = -conj(W^k),
 // X[N-k] = conj(E - W^k O).
 template <int N>
-CORR_HD CORR_FI void r2c_pair(const float2 *lds, const float2 *wl, int k, float2 &xk, float2 &xm) {
+CORR_HD CORR_FI void r2c_pair(const float2 *lds, float2 w, int k, float2 &xk, float2 &xm) {
     const float2 a = lds[pad16(k)];
     const float2 b = lds[pad16(N - k)];
     const float er = 0.5f * (a.x + b.x), ei = 0.5f * (a.y - b.y);
     const float orr = 0.5f * (a.y + b.y), oi = -0.5f * (a.x - b.x);
-    const float2 w = wl[(unsigned)k];
     const float tr = w.x * orr - w.y * oi, ti = w.x * oi + w.y * orr;
     xk = make_float2(er + tr, ei + ti);
     xm = make_float2(er - tr, ti - ei);
@@ -284,18 +317,22 @@ CORR_HD CORR_FI void r2c_pair(const float2 *lds, const float2 *wl, int k, float2
 // The R2C epilogue of one workgroup: every retained bin k = 0..N-1 exactly once,
 // thread j taking k = j + q*N/16 (q < 8) with its mirror N - k, thread 0 also
 // the self-mirrored bin N/2 (bin 0's mirror is the Nyquist bin, never formed).
-// emit(k, X) stores one bin.
+// W^k = W^j W^(q N/16) with W^(q N/16) = exp(-i pi q/16), a constant: one
+// table read per thread. emit(k, X) stores one bin.
 template <int N, class Emit>
 CORR_HD CORR_FI void r2c_epilogue(int j, const float2 *lds, const float2 *wl, Emit &&emit) {
     constexpr int NT = N / 16;
+    const float2 wj = wl[(unsigned)j];
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
         const int k = j + q * NT;
         if (k == 0) {
             emit(0, r2c_bin<N>(lds, wl, 0));
         } else {
+            // exp(-i pi q/16) = cos(2 pi q/32) - i sin(2 pi q/32)
+            const float2 w = q == 0 ? wj : cmul(wj, make_float2(cos32(q), -cos32((q + 24) & 31)));
             float2 xk, xm;
-            r2c_pair<N>(lds, wl, k, xk, xm);
+            r2c_pair<N>(lds, w, k, xk, xm);
             emit(k, xk);
             emit(N - k, xm);
         }
