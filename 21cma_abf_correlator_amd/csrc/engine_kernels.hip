@@ -151,8 +151,10 @@ __global__ void __launch_bounds__(kScaleBlock) scale_kernel(float2 *acc, size_t 
 // MFMA X-engine (FP16 / int8 forms)
 // grid (n_freq/16), block 512; dynamic LDS xmfma::lds_bytes(n_station)
 // ---------------------------------------------------------------------------
+// Up to 24 inputs (3 row tiles) two workgroups share a CU (LDS <= 62 KB each),
+// which holds each lane to 128 registers.
 template <int FORM, int NRT>
-__global__ void __launch_bounds__(xmfma::kThreads)
+__global__ void __launch_bounds__(xmfma::kThreads, NRT <= 3 ? 2 : 1)
 xengine_mfma_kernel(const void *__restrict__ spectra, size_t ys, int nf, int frames, int n_station,
                     int n_tiles, void *__restrict__ integ) {
     extern __shared__ uint4 lds_words[];

@@ -132,9 +132,24 @@ CORR_HD CORR_FI void dft(float2 *v) {
     unscramble<R, 0>(v);
 }
 
-// LDS index padding: one spare float2 every 16, so that a wave writing with a
-// stride of 16 elements (the first Stockham pass) spreads over all banks.
+// LDS layouts of the exchange buffer. A pass whose lanes write with a stride
+// (the first pass, stride 16, and the NS = 16 pass, runs of 16) writes the
+// padded layout, one spare float2 after every 16, so that a wave's writes spread
+// over all banks; a pass whose lanes write contiguous runs of 64 or more (NS >=
+// 64) writes the plain layout. Every read uses the layout its data was written
+// in. Offsets that are multiples of 16 elements map linearly in both layouts,
+// so each access is one base address plus a compile-time offset.
 CORR_HD CORR_FI int pad16(int a) { return a + (a >> 4); }
+
+template <bool PAD>
+CORR_HD CORR_FI int lds_index(int a) { return PAD ? pad16(a) : a; }
+
+// Elements of an offset that is a multiple of 16, in the layout.
+template <bool PAD>
+CORR_HD constexpr int lds_span(int a16) { return PAD ? a16 + a16 / 16 : a16; }
+
+// The layout written by a pass whose butterflies combine NS points.
+CORR_HD constexpr bool pad_for(int ns) { return ns < 64; }
 
 template <int N>
 struct Sizes {
@@ -177,6 +192,9 @@ CORR_HD CORR_FI void stockham_pass(float2 (&v)[16], float2 *lds, const float2 *t
     constexpr int NB = 16 / R;
     constexpr int STRIDE = N / R;
     constexpr int TWSTEP = N / (NS * R);
+    constexpr bool PIN = pad_for(NS / 16 > 0 ? NS / 16 : 1);  // layout of the previous pass's writes
+    constexpr bool POUT = pad_for(NS);
+    static_assert(STRIDE % 16 == 0 && NS % 16 == 0, "offsets are whole 16-element runs");
     float2 w[NB][R - 1];
     if constexpr (NS * R == N && NB > 1) {
         // Last pass: k = j + q*N/16 exactly, so group q's twiddles are group 0's
@@ -193,11 +211,12 @@ CORR_HD CORR_FI void stockham_pass(float2 (&v)[16], float2 *lds, const float2 *t
 #pragma unroll
         for (int q = 0; q < NB; ++q) group_twiddles<R, TWSTEP>(tw, (j + q * NT) & (NS - 1), w[q]);
     }
+    {
+        const float2 *src = lds + lds_index<PIN>(j);
 #pragma unroll
-    for (int q = 0; q < NB; ++q) {
-        const int jj = j + q * NT;
+        for (int q = 0; q < NB; ++q)
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[q * R + r] = lds[pad16(jj + r * STRIDE)];
+            for (int r = 0; r < R; ++r) v[q * R + r] = src[lds_span<PIN>(q * NT + r * STRIDE)];
     }
     sync();
 #pragma unroll
@@ -207,9 +226,9 @@ CORR_HD CORR_FI void stockham_pass(float2 (&v)[16], float2 *lds, const float2 *t
 #pragma unroll
         for (int r = 1; r < R; ++r) v[q * R + r] = cmul(v[q * R + r], w[q][r - 1]);
         dft<R>(&v[q * R]);
-        const int base = (jj - k) * R + k;
+        float2 *dst = lds + lds_index<POUT>((jj - k) * R + k);
 #pragma unroll
-        for (int r = 0; r < R; ++r) lds[pad16(base + r * NS)] = v[q * R + r];
+        for (int r = 0; r < R; ++r) dst[lds_span<POUT>(r * NS)] = v[q * R + r];
     }
     sync();
 }
@@ -267,10 +286,11 @@ CORR_HD CORR_FI bool frame_present(const int32_t *valid, size_t valid_stride, in
 // order, padded); on return `lds` holds Z after a final barrier.
 template <int LOG2N, class Sync>
 CORR_HD CORR_FI void fft_from_values(int j, float2 (&v)[16], float2 *lds, Sync &sync, const float2 *tw) {
+    static_assert(LOG2N >= 9 && LOG2N <= 14, "the plain-layout R2C reads need a last pass with NS >= 256");
     // first pass: radix 16, NS = 1, inputs z[j + r*N/16] already in registers
     dft<16>(v);
 #pragma unroll
-    for (int r = 0; r < 16; ++r) lds[pad16(j * 16 + r)] = v[r];
+    for (int r = 0; r < 16; ++r) lds[17 * j + r] = v[r];  // padded layout: pad16(16 j + r)
     sync();
     stockham_rest<LOG2N, 4>(v, lds, tw, j, sync);
 }
@@ -289,11 +309,12 @@ CORR_HD CORR_FI void complex_fft_block(int j, int f, int st, float2 *lds, Sync &
     fft_from_values<LOG2N>(j, v, lds, sync, tw);
 }
 
-// R2C bin k from Z in lds: (Z[k] + conj(Z[N-k]))/2 + W^k (Z[k] - conj(Z[N-k]))/(2i).
+// R2C bin k from Z in lds (plain layout: the last pass of every supported
+// length combines NS >= 256 points): (Z[k] + conj(Z[N-k]))/2 + W^k (Z[k] - conj(Z[N-k]))/(2i).
 template <int N>
 CORR_HD CORR_FI float2 r2c_bin(const float2 *lds, const float2 *wl, int k) {
-    const float2 a = lds[pad16(k)];
-    const float2 b = lds[pad16((N - k) & (N - 1))];
+    const float2 a = lds[k];
+    const float2 b = lds[(N - k) & (N - 1)];
     const float er = 0.5f * (a.x + b.x), ei = 0.5f * (a.y - b.y);
     const float orr = 0.5f * (a.y + b.y), oi = -0.5f * (a.x - b.x);
     const float2 w = wl[(unsigned)k];
@@ -305,8 +326,8 @@ CORR_HD CORR_FI float2 r2c_bin(const float2 *lds, const float2 *wl, int k) {
 // X[N-k] = conj(E - W^k O).
 template <int N>
 CORR_HD CORR_FI void r2c_pair(const float2 *lds, float2 w, int k, float2 &xk, float2 &xm) {
-    const float2 a = lds[pad16(k)];
-    const float2 b = lds[pad16(N - k)];
+    const float2 a = lds[k];
+    const float2 b = lds[N - k];
     const float er = 0.5f * (a.x + b.x), ei = 0.5f * (a.y - b.y);
     const float orr = 0.5f * (a.y + b.y), oi = -0.5f * (a.x - b.x);
     const float tr = w.x * orr - w.y * oi, ti = w.x * oi + w.y * orr;

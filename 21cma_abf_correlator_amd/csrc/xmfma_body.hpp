@@ -110,44 +110,76 @@ CORR_HD inline void neumaier(float &s, float &e, float x) {
 }
 
 // Staging of one item: 16 B (FP16) per frame for 8 frames, or 8 B (int8) per
-// frame for 16 frames, of one (input, channel quad); the 8 transposed rows of
-// 16 B go to lds[(4q+m)][2s+p][16*o .. 16*o+15] for channel m of the quad and
-// part p (0 = real, 1 = imaginary).
+// frame for 16 frames, of one (input, channel quad), held in 32 registers
+// between its load and its store; the store transposes them into 8 rows of
+// 16 B at lds[(4q+m)][2s+p][16*o .. 16*o+15] for channel m of the quad and part
+// p (0 = real, 1 = imaginary). Frames at or past `frames` load as zeros.
+struct Stage {
+    uint32_t w[32];
+};
+
 template <int FORM>
-CORR_HD inline void stage_item(int item, uint8_t *lds, const void *spectra, size_t ys, int nf,
-                               int frames, int f0, int c0, size_t cpitch);
+CORR_HD inline void load_item(int item, const void *spectra, size_t ys, int nf, int frames, int f0, int c0,
+                              Stage &st);
 
 template <>
-CORR_HD inline void stage_item<1>(int item, uint8_t *lds, const void *spectra, size_t ys, int nf,
-                                  int frames, int f0, int c0, size_t cpitch) {
+CORR_HD inline void load_item<1>(int item, const void *spectra, size_t ys, int nf, int frames, int f0, int c0,
+                                 Stage &st) {
     const int q = item & 3, o = (item >> 2) & 3, s = item >> 4;
     const uint32_t *y = static_cast<const uint32_t *>(spectra);
-    uint32_t w[8][4];
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
         const int f = f0 + 8 * o + i;
         if (f < frames) {
             const uint32_t *src = y + (size_t)s * ys + (size_t)f * (size_t)nf + (size_t)(c0 + 4 * q);
             const uint4 v = *reinterpret_cast<const uint4 *>(src);
-            w[i][0] = v.x;
-            w[i][1] = v.y;
-            w[i][2] = v.z;
-            w[i][3] = v.w;
+            st.w[4 * i + 0] = v.x;
+            st.w[4 * i + 1] = v.y;
+            st.w[4 * i + 2] = v.z;
+            st.w[4 * i + 3] = v.w;
         } else {
-            w[i][0] = w[i][1] = w[i][2] = w[i][3] = 0u;
+            st.w[4 * i + 0] = st.w[4 * i + 1] = st.w[4 * i + 2] = st.w[4 * i + 3] = 0u;
         }
     }
+}
+
+template <>
+CORR_HD inline void load_item<2>(int item, const void *spectra, size_t ys, int nf, int frames, int f0, int c0,
+                                 Stage &st) {
+    const int q = item & 3, o = (item >> 2) & 3, s = item >> 4;
+    const uint16_t *y = static_cast<const uint16_t *>(spectra);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int f = f0 + 16 * o + i;
+        if (f < frames) {
+            const uint16_t *src = y + (size_t)s * ys + (size_t)f * (size_t)nf + (size_t)(c0 + 4 * q);
+            const uint2 v = *reinterpret_cast<const uint2 *>(src);
+            st.w[2 * i + 0] = v.x;  // channels 0, 1 of the quad: re0 im0 re1 im1
+            st.w[2 * i + 1] = v.y;  // channels 2, 3
+        } else {
+            st.w[2 * i + 0] = st.w[2 * i + 1] = 0u;
+        }
+    }
+}
+
+template <int FORM>
+CORR_HD inline void store_item(int item, uint8_t *lds, size_t cpitch, const Stage &st);
+
+template <>
+CORR_HD inline void store_item<1>(int item, uint8_t *lds, size_t cpitch, const Stage &st) {
+    const int q = item & 3, o = (item >> 2) & 3, s = item >> 4;
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
+        const uint32_t *w = st.w + m;  // w[4 i] = frame i's word of channel m
         uint4 re, im;
-        re.x = (w[0][m] & 0xFFFFu) | (w[1][m] << 16);
-        re.y = (w[2][m] & 0xFFFFu) | (w[3][m] << 16);
-        re.z = (w[4][m] & 0xFFFFu) | (w[5][m] << 16);
-        re.w = (w[6][m] & 0xFFFFu) | (w[7][m] << 16);
-        im.x = (w[0][m] >> 16) | (w[1][m] & 0xFFFF0000u);
-        im.y = (w[2][m] >> 16) | (w[3][m] & 0xFFFF0000u);
-        im.z = (w[4][m] >> 16) | (w[5][m] & 0xFFFF0000u);
-        im.w = (w[6][m] >> 16) | (w[7][m] & 0xFFFF0000u);
+        re.x = (w[0] & 0xFFFFu) | (w[4] << 16);
+        re.y = (w[8] & 0xFFFFu) | (w[12] << 16);
+        re.z = (w[16] & 0xFFFFu) | (w[20] << 16);
+        re.w = (w[24] & 0xFFFFu) | (w[28] << 16);
+        im.x = (w[0] >> 16) | (w[4] & 0xFFFF0000u);
+        im.y = (w[8] >> 16) | (w[12] & 0xFFFF0000u);
+        im.z = (w[16] >> 16) | (w[20] & 0xFFFF0000u);
+        im.w = (w[24] >> 16) | (w[28] & 0xFFFF0000u);
         uint8_t *row = lds + (size_t)(4 * q + m) * cpitch + (size_t)(2 * s) * kRowBytes + 16 * o;
         *reinterpret_cast<uint4 *>(row) = re;
         *reinterpret_cast<uint4 *>(row + kRowBytes) = im;
@@ -155,23 +187,8 @@ CORR_HD inline void stage_item<1>(int item, uint8_t *lds, const void *spectra, s
 }
 
 template <>
-CORR_HD inline void stage_item<2>(int item, uint8_t *lds, const void *spectra, size_t ys, int nf,
-                                  int frames, int f0, int c0, size_t cpitch) {
+CORR_HD inline void store_item<2>(int item, uint8_t *lds, size_t cpitch, const Stage &st) {
     const int q = item & 3, o = (item >> 2) & 3, s = item >> 4;
-    const uint16_t *y = static_cast<const uint16_t *>(spectra);
-    uint32_t w[16][2];
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const int f = f0 + 16 * o + i;
-        if (f < frames) {
-            const uint16_t *src = y + (size_t)s * ys + (size_t)f * (size_t)nf + (size_t)(c0 + 4 * q);
-            const uint2 v = *reinterpret_cast<const uint2 *>(src);
-            w[i][0] = v.x;  // channels 0, 1 of the quad: re0 im0 re1 im1
-            w[i][1] = v.y;  // channels 2, 3
-        } else {
-            w[i][0] = w[i][1] = 0u;
-        }
-    }
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
 #pragma unroll
@@ -183,7 +200,7 @@ CORR_HD inline void stage_item<2>(int item, uint8_t *lds, const void *spectra, s
             for (int k = 0; k < 4; ++k) {
                 uint32_t acc = 0;
 #pragma unroll
-                for (int b = 0; b < 4; ++b) acc |= ((w[4 * k + b][m >> 1] >> sh) & 0xFFu) << (8 * b);
+                for (int b = 0; b < 4; ++b) acc |= ((st.w[2 * (4 * k + b) + (m >> 1)] >> sh) & 0xFFu) << (8 * b);
                 rw[k] = acc;
             }
             uint8_t *row = lds + (size_t)(4 * q + m) * cpitch + (size_t)(2 * s + p) * kRowBytes + 16 * o;
@@ -225,10 +242,33 @@ CORR_HD inline void xengine_mfma_block(int tid, int bx, int n_tiles, uint8_t *ld
 #pragma unroll
             for (int v = 0; v < 4; ++v) acc[ch][p].v[v] = (acc_t)0;
 
+    // Items of this lane: tid, tid + kThreads (the second only for > 32 inputs).
+    // The next step's loads are issued right after this step's stores, into
+    // the same registers, so they are in flight during the MFMA phase.
+    // With two items per lane (more than 32 inputs) the registers of a second
+    // staged step do not fit beside the accumulators: those load each step in turn.
+    constexpr int IPT = (16 * 8 * NRT + kThreads - 1) / kThreads;
+    constexpr bool kAhead = IPT == 1;
+    Stage stg[IPT];
+    if constexpr (kAhead) {
+        if (tid < n_items) load_item<FORM>(tid, spectra, ys, nf, frames, 0, c0, stg[0]);
+    }
+
     for (int f0 = 0; f0 < frames; f0 += T::kFramesPerStep) {
         sync();
-        for (int item = tid; item < n_items; item += kThreads)
-            stage_item<FORM>(item, lds, spectra, ys, nf, frames, f0, c0, cpitch);
+        if constexpr (kAhead) {
+            if (tid < n_items) {
+                store_item<FORM>(tid, lds, cpitch, stg[0]);
+                if (f0 + T::kFramesPerStep < frames)
+                    load_item<FORM>(tid, spectra, ys, nf, frames, f0 + T::kFramesPerStep, c0, stg[0]);
+            }
+        } else {
+#pragma unroll 1
+            for (int item = tid; item < n_items; item += kThreads) {
+                load_item<FORM>(item, spectra, ys, nf, frames, f0, c0, stg[0]);
+                store_item<FORM>(item, lds, cpitch, stg[0]);
+            }
+        }
         sync();
 #pragma unroll
         for (int ch = 0; ch < kChanPerWave; ++ch) {
