@@ -68,13 +68,17 @@ struct MfmaOps {
 // F-engine, complex64 output
 // grid (frames, stations), block N/16 threads, N = n_freq = 2^LOG2N
 // ---------------------------------------------------------------------------
+// At n_freq 16384 the transform exchanges through a half buffer (69.6 KB of
+// LDS) so that two 1024-lane workgroups share a CU: 8 waves per SIMD, 64
+// registers per lane.
 template <int LOG2N>
 __global__ void __launch_bounds__((1 << LOG2N) / 16)
+__attribute__((amdgpu_waves_per_eu(LOG2N == 14 ? 8 : 1)))
 fengine_fp32_kernel(const int8_t *__restrict__ in, size_t station_pitch,
                     const int32_t *__restrict__ valid, size_t valid_stride,
                     const float2 *__restrict__ tw, const float2 *__restrict__ wl,
                     float2 *__restrict__ out, size_t out_station_stride) {
-    __shared__ float2 lds[fft::Sizes<(1 << LOG2N)>::kLds];
+    __shared__ float2 lds[fft::fengine_lds_elems<LOG2N>()];
     DeviceSync sync;
     fft::fengine_fp32_block<LOG2N>((int)threadIdx.x, (int)blockIdx.x, (int)blockIdx.y, lds, sync, in,
                                    station_pitch, valid, valid_stride, tw, wl, out,
@@ -85,12 +89,13 @@ fengine_fp32_kernel(const int8_t *__restrict__ in, size_t station_pitch,
 // and conversion per bin; output [station][frame][N] packed complex.
 template <int LOG2N, int FORM>
 __global__ void __launch_bounds__((1 << LOG2N) / 16)
+__attribute__((amdgpu_waves_per_eu(LOG2N == 14 ? 8 : 1)))
 fengine_form_kernel(const int8_t *__restrict__ in, size_t station_pitch,
                     const int32_t *__restrict__ valid, size_t valid_stride,
                     const float2 *__restrict__ tw, const float2 *__restrict__ wl,
                     const float *__restrict__ gain, void *__restrict__ out,
                     size_t out_station_stride) {
-    __shared__ float2 lds[fft::Sizes<(1 << LOG2N)>::kLds];
+    __shared__ float2 lds[fft::fengine_lds_elems<LOG2N>()];
     DeviceSync sync;
     fft::fengine_form_block<LOG2N, FORM>((int)threadIdx.x, (int)blockIdx.x, (int)blockIdx.y, lds,
                                          sync, in, station_pitch, valid, valid_stride, tw, wl, gain,

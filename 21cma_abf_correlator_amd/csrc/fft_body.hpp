@@ -167,12 +167,16 @@ CORR_HD constexpr int high_bit(int r) { return r >= 16 ? 16 : r >= 8 ? 8 : r >= 
 // the F-engine's costliest memory operation (a wave's lanes read scattered
 // entries), so this trades eleven of them per radix-16 group for complex products.
 template <int R, int TWSTEP>
-CORR_HD CORR_FI void group_twiddles(const float2 *tw, int k, float2 (&w)[R - 1]) {
+CORR_HD CORR_FI void group_twiddle_reads(const float2 *tw, int k, float2 (&w)[R - 1]) {
 #pragma unroll
     for (int r = 1; r < R; ++r) {
         if ((r & (r - 1)) == 0)
             w[r - 1] = tw[(unsigned)(r * k * TWSTEP)];
     }
+}
+
+template <int R>
+CORR_HD CORR_FI void group_twiddle_products(float2 (&w)[R - 1]) {
 #pragma unroll
     for (int r = 3; r < R; ++r) {
         if ((r & (r - 1)) != 0) {
@@ -180,6 +184,12 @@ CORR_HD CORR_FI void group_twiddles(const float2 *tw, int k, float2 (&w)[R - 1])
             w[r - 1] = cmul(w[h - 1], w[r - h - 1]);
         }
     }
+}
+
+template <int R, int TWSTEP>
+CORR_HD CORR_FI void group_twiddles(const float2 *tw, int k, float2 (&w)[R - 1]) {
+    group_twiddle_reads<R, TWSTEP>(tw, k, w);
+    group_twiddle_products<R>(w);
 }
 
 // One Stockham pass of radix R over N points, NS = points already combined,
@@ -362,15 +372,21 @@ CORR_HD CORR_FI void r2c_epilogue(int j, const float2 *lds, const float2 *wl, Em
 }
 
 // Whole F-engine workgroup, complex64 output [station][frame][N].
+template <int LOG2N, class Sync, class Emit>
+CORR_HD CORR_FI void fengine_frame(int j, int f, int st, float2 *lds, Sync &sync, const int8_t *in,
+                                   size_t station_pitch, const int32_t *valid, size_t valid_stride,
+                                   const float2 *tw, const float2 *wl, Emit &&emit);
+
+// Whole F-engine workgroup, complex64 output [station][frame][N].
 template <int LOG2N, class Sync>
 CORR_HD CORR_FI void fengine_fp32_block(int j, int f, int st, float2 *lds, Sync &sync, const int8_t *in,
                                        size_t station_pitch, const int32_t *valid, size_t valid_stride,
                                        const float2 *tw, const float2 *wl, float2 *out,
                                        size_t out_station_stride) {
     constexpr int N = 1 << LOG2N;
-    complex_fft_block<LOG2N>(j, f, st, lds, sync, in, station_pitch, valid, valid_stride, tw);
     float2 *dst = out + (size_t)st * out_station_stride + (size_t)f * (size_t)N;
-    r2c_epilogue<N>(j, lds, wl, [&](int k, float2 x) { dst[(unsigned)k] = x; });
+    fengine_frame<LOG2N>(j, f, st, lds, sync, in, station_pitch, valid, valid_stride, tw, wl,
+                         [&](int k, float2 x) { dst[(unsigned)k] = x; });
 }
 
 // Whole F-engine workgroup for the FP16 (FORM 1) or int8 (FORM 2) form: bin k
@@ -382,15 +398,229 @@ CORR_HD CORR_FI void fengine_form_block(int j, int f, int st, float2 *lds, Sync 
                                        const float2 *tw, const float2 *wl, const float *gain,
                                        void *out, size_t out_station_stride) {
     constexpr int N = 1 << LOG2N;
-    complex_fft_block<LOG2N>(j, f, st, lds, sync, in, station_pitch, valid, valid_stride, tw);
     const float *g = gain + (size_t)st * N;
     const size_t at = (size_t)st * out_station_stride + (size_t)f * (size_t)N;
     if constexpr (FORM == 1) {
         uint32_t *dst = static_cast<uint32_t *>(out) + at;
-        r2c_epilogue<N>(j, lds, wl, [&](int k, float2 x) { dst[(unsigned)k] = forms::pack_fp16(x, g[(unsigned)k]); });
+        fengine_frame<LOG2N>(j, f, st, lds, sync, in, station_pitch, valid, valid_stride, tw, wl,
+                             [&](int k, float2 x) { dst[(unsigned)k] = forms::pack_fp16(x, g[(unsigned)k]); });
     } else {
         uint16_t *dst = static_cast<uint16_t *>(out) + at;
-        r2c_epilogue<N>(j, lds, wl, [&](int k, float2 x) { dst[(unsigned)k] = forms::pack_int8(x, g[(unsigned)k]); });
+        fengine_frame<LOG2N>(j, f, st, lds, sync, in, station_pitch, valid, valid_stride, tw, wl,
+                             [&](int k, float2 x) { dst[(unsigned)k] = forms::pack_int8(x, g[(unsigned)k]); });
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Half-buffer exchanges. At N = 16384 a full exchange buffer (136 KB) leaves
+// room for one workgroup per CU, whose barriers and first HBM read then idle
+// the CU. Here every exchange goes through a buffer of N/2 points in two rounds
+// (write the values bound for one half, barrier, read the values taken from
+// it), so two workgroups share a CU and each hides the other's stalls. Which
+// half a value belongs to is fixed by the lane's wave or by the radix digit,
+// never data-dependent, so no wave diverges.
+// ---------------------------------------------------------------------------
+template <int N>
+struct HalfSizes {
+    static constexpr int kLds = N / 2 + N / 32;  // padded half buffer, float2
+};
+
+// Write the outputs of pass (R, NS) held in v that fall in half h of [0, N).
+// R = 16 (one group per lane): the half is set by the lane; the last pass
+// (NS*R = N): by the digit r.
+template <int N, int R, int NS>
+CORR_HD CORR_FI void half_write(const float2 (&v)[16], float2 *lds, int j, int h) {
+    constexpr int NT = N / 16;
+    constexpr int NB = 16 / R;
+    constexpr bool POUT = pad_for(NS);
+    if constexpr (R == 16) {
+        const int k = j & (NS - 1);
+        const int base = (j - k) * 16 + k;  // < N/2 exactly when the lane's writes are
+        if ((base >= N / 2) == (h == 1)) {
+            float2 *dst = lds + lds_index<POUT>(base - h * (N / 2));
+#pragma unroll
+            for (int r = 0; r < 16; ++r) dst[NS == 1 ? r : lds_span<POUT>(r * NS)] = v[r];
+        }
+    } else {
+        static_assert(NS * R == N, "a pass of radix below 16 is the last pass");
+#pragma unroll
+        for (int q = 0; q < NB; ++q) {
+            float2 *dst = lds + lds_index<POUT>(j + q * NT);
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                if ((2 * r >= R) == (h == 1)) dst[lds_span<POUT>(r * NS - h * (N / 2))] = v[q * R + r];
+        }
+    }
+}
+
+// Read the inputs of the pass of radix R (stride N/R) that lie in half h.
+template <int N, int R, bool PIN>
+CORR_HD CORR_FI void half_read(float2 (&v)[16], const float2 *lds, int j, int h) {
+    constexpr int NT = N / 16;
+    constexpr int NB = 16 / R;
+    constexpr int STRIDE = N / R;
+    const float2 *src = lds + lds_index<PIN>(j);
+#pragma unroll
+    for (int q = 0; q < NB; ++q)
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if ((2 * r >= R) == (h == 1)) v[q * R + r] = src[lds_span<PIN>(q * NT + r * STRIDE - h * (N / 2))];
+}
+
+// One exchange from pass (R0, NS0)'s outputs to the inputs of the next pass of
+// radix R1, through the half buffer. `first` skips the barrier that protects
+// the previous exchange's reads.
+template <int N, int R0, int NS0, int R1, class Sync>
+CORR_HD CORR_FI void half_exchange(float2 (&v)[16], float2 *lds, int j, Sync &sync, bool first) {
+    float2 in[16];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        if (h == 1 || !first) sync();
+        half_write<N, R0, NS0>(v, lds, j, h);
+        sync();
+        half_read<N, R1, pad_for(NS0)>(in, lds, j, h);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = in[r];
+}
+
+// Pass (R, NS) arithmetic on inputs already in registers: twiddles, DFT.
+template <int N, int R, int NS>
+CORR_HD CORR_FI void pass_compute(float2 (&v)[16], const float2 (&w)[16 / R][R - 1]) {
+    constexpr int NB = 16 / R;
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[q * R + r] = cmul(v[q * R + r], w[q][r - 1]);
+        dft<R>(&v[q * R]);
+    }
+}
+
+// A pass's twiddles in two steps: the table reads (issued before the
+// exchange, so that they are in flight during it) and the products (after it).
+template <int N, int R, int NS>
+CORR_HD CORR_FI void pass_twiddle_reads(const float2 *tw, int j, float2 (&w)[16 / R][R - 1]) {
+    constexpr int NT = N / 16;
+    constexpr int NB = 16 / R;
+    constexpr int TWSTEP = N / (NS * R);
+    if constexpr (NS * R == N && NB > 1) {
+        group_twiddle_reads<R, TWSTEP>(tw, j, w[0]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < NB; ++q) group_twiddle_reads<R, TWSTEP>(tw, (j + q * NT) & (NS - 1), w[q]);
+    }
+}
+
+template <int N, int R, int NS>
+CORR_HD CORR_FI void pass_twiddle_products(float2 (&w)[16 / R][R - 1]) {
+    constexpr int NB = 16 / R;
+    if constexpr (NS * R == N && NB > 1) {
+        group_twiddle_products<R>(w[0]);
+#pragma unroll
+        for (int q = 1; q < NB; ++q)
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                const int m = (2 * r * q) & 31;
+                w[q][r - 1] = cmul(w[0][r - 1], make_float2(cos32(m), -cos32((m + 24) & 31)));
+            }
+    } else {
+#pragma unroll
+        for (int q = 0; q < NB; ++q) group_twiddle_products<R>(w[q]);
+    }
+}
+
+// The N = 16384 transform (passes 16, 16, 16, 4) and the R2C epilogue, all
+// exchanges through the half buffer. emit(k, X) stores one bin.
+template <class Sync, class Emit>
+CORR_HD CORR_FI void fengine_half_16384(int j, float2 (&v)[16], float2 *lds, Sync &sync, const float2 *tw,
+                                        const float2 *wl, Emit &&emit) {
+    constexpr int N = 16384;
+    constexpr int NT = N / 16;
+    dft<16>(v);  // first pass: radix 16, NS = 1
+    {
+        float2 w[1][15];
+        pass_twiddle_reads<N, 16, 16>(tw, j, w);
+        half_exchange<N, 16, 1, 16>(v, lds, j, sync, true);
+        pass_twiddle_products<N, 16, 16>(w);
+        pass_compute<N, 16, 16>(v, w);
+    }
+    {
+        float2 w[1][15];
+        pass_twiddle_reads<N, 16, 256>(tw, j, w);
+        half_exchange<N, 16, 16, 16>(v, lds, j, sync, false);
+        pass_twiddle_products<N, 16, 256>(w);
+        pass_compute<N, 16, 256>(v, w);
+    }
+    {
+        float2 w[4][3];
+        pass_twiddle_reads<N, 4, 4096>(tw, j, w);
+        half_exchange<N, 16, 256, 4>(v, lds, j, sync, false);
+        pass_twiddle_products<N, 4, 4096>(w);
+        pass_compute<N, 4, 4096>(v, w);
+    }
+    // R2C: Z[k] (k < N/2) from half 0, Z[N-k] from half 1 (the last pass wrote
+    // the plain layout).
+    const float2 wj = wl[(unsigned)j];
+    float2 a[8], b[8], zh = make_float2(0.0f, 0.0f);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        sync();
+        half_write<N, 4, 4096>(v, lds, j, h);
+        sync();
+        if (h == 0) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) a[q] = lds[j + q * NT];
+        } else {
+#pragma unroll
+            for (int q = 0; q < 8; ++q) b[q] = lds[(N / 2 - j - q * NT) & (N / 2 - 1)];
+            if (j == 0) zh = lds[0];  // Z[N/2]
+        }
+    }
+    // b[0] of lane 0 is Z[N/2] (read at index 0 of half 1); bin 0 pairs with
+    // itself: X[0] = Re Z[0] + Im Z[0].
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int k = j + q * NT;
+        const float2 w = q == 0 ? wj : cmul(wj, make_float2(cos32(q), -cos32((q + 24) & 31)));
+        if (k == 0) {
+            emit(0, make_float2(a[0].x + a[0].y, 0.0f));
+        } else {
+            const float er = 0.5f * (a[q].x + b[q].x), ei = 0.5f * (a[q].y - b[q].y);
+            const float orr = 0.5f * (a[q].y + b[q].y), oi = -0.5f * (a[q].x - b[q].x);
+            const float tr = w.x * orr - w.y * oi, ti = w.x * oi + w.y * orr;
+            emit(k, make_float2(er + tr, ei + ti));
+            emit(N - k, make_float2(er - tr, ti - ei));
+        }
+    }
+    if (j == 0) {
+        // X[N/2] = (Z + conj Z)/2 + W^(N/2) (Z - conj Z)/(2i) with W^(N/2) = -i
+        emit(N / 2, make_float2(zh.x, -zh.y));
+    }
+}
+
+// LDS elements (float2) of the F-engine workgroup at N = 2^LOG2N.
+template <int LOG2N>
+CORR_HD constexpr int fengine_lds_elems() {
+    return LOG2N == 14 ? HalfSizes<(1 << LOG2N)>::kLds : Sizes<(1 << LOG2N)>::kLds;
+}
+
+// One frame of one station through the F-engine: N = 16384 exchanges through
+// the half buffer, shorter transforms through the full one.
+template <int LOG2N, class Sync, class Emit>
+CORR_HD CORR_FI void fengine_frame(int j, int f, int st, float2 *lds, Sync &sync, const int8_t *in,
+                                   size_t station_pitch, const int32_t *valid, size_t valid_stride,
+                                   const float2 *tw, const float2 *wl, Emit &&emit) {
+    constexpr int N = 1 << LOG2N;
+    if constexpr (LOG2N == 14) {
+        char2 raw[16];
+        load_frame<LOG2N>(j, in, station_pitch, f, st, frame_present(valid, valid_stride, f, st), raw);
+        float2 v[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) v[r] = make_float2((float)raw[r].x, (float)raw[r].y);
+        fengine_half_16384(j, v, lds, sync, tw, wl, emit);
+    } else {
+        complex_fft_block<LOG2N>(j, f, st, lds, sync, in, station_pitch, valid, valid_stride, tw);
+        r2c_epilogue<N>(j, lds, wl, emit);
     }
 }
 

@@ -42,7 +42,7 @@ hipError_t fengine_t(const int8_t *in, size_t station_pitch, const int32_t *vali
                      const float2 *tw, const float2 *wl, float2 *out, size_t out_station_stride,
                      int frames, int n_station) {
     constexpr int N = 1 << LOG2N;
-    run_grid(frames, n_station, N / 16, fft::Sizes<N>::kLds * sizeof(float2),
+    run_grid(frames, n_station, N / 16, fft::fengine_lds_elems<LOG2N>() * sizeof(float2),
              [=](int j, int bx, int by, char *lds, HostSync &sync) {
                  fft::fengine_fp32_block<LOG2N>(j, bx, by, reinterpret_cast<float2 *>(lds), sync, in,
                                                 station_pitch, valid, valid_stride, tw, wl, out,
@@ -56,7 +56,7 @@ hipError_t fengine_form_t(const int8_t *in, size_t station_pitch, const int32_t 
                           const float2 *tw, const float2 *wl, const float *gain, void *out,
                           size_t out_station_stride, int frames, int n_station) {
     constexpr int N = 1 << LOG2N;
-    run_grid(frames, n_station, N / 16, fft::Sizes<N>::kLds * sizeof(float2),
+    run_grid(frames, n_station, N / 16, fft::fengine_lds_elems<LOG2N>() * sizeof(float2),
              [=](int j, int bx, int by, char *lds, HostSync &sync) {
                  fft::fengine_form_block<LOG2N, FORM>(j, bx, by, reinterpret_cast<float2 *>(lds), sync, in,
                                                       station_pitch, valid, valid_stride, tw, wl, gain, out,

@@ -25,7 +25,7 @@ static void run_block(int f, int st, const int8_t *in, size_t pitch, const int32
                       size_t ostride) {
     constexpr int N = 1 << LOG2N;
     constexpr int NT = N / 16;
-    float2 *lds = new float2[Sizes<N>::kLds];  // heap, so ASan sees its bounds
+    float2 *lds = new float2[fengine_lds_elems<LOG2N>()];  // heap, so ASan sees its bounds
     std::barrier<> bar(NT);
     std::vector<std::thread> th;
     th.reserve(NT);
@@ -45,7 +45,7 @@ static void run_form_block(int f, int st, const int8_t *in, size_t pitch, const 
                            void *out, size_t ostride) {
     constexpr int N = 1 << LOG2N;
     constexpr int NT = N / 16;
-    float2 *lds = new float2[Sizes<N>::kLds];
+    float2 *lds = new float2[fengine_lds_elems<LOG2N>()];
     std::barrier<> bar(NT);
     std::vector<std::thread> th;
     th.reserve(NT);
