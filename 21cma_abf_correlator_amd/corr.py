@@ -89,7 +89,7 @@ ABI_SYMBOLS = (
 ENGINE_SYMBOLS = (
     "corrx_engine_create", "corrx_engine_destroy", "corrx_fengine", "corrx_xengine",
     "corrx_scale", "corrx_spectra", "corrx_integrator_bytes", "corrx_dump", "corrx_calibrate",
-    "corrx_set_gains", "corrx_gains",
+    "corrx_set_gains", "corrx_gains", "corrx_set_xtile", "corrx_get_xtile",
 )
 
 _lib = None
@@ -172,6 +172,8 @@ def declare(L):
         "corrx_calibrate": ([vp, vp, sz, vp, sz, i32, i32, vp], st),
         "corrx_set_gains": ([vp, vp, vp], st),
         "corrx_gains": ([vp, ctypes.POINTER(vp)], st),
+        "corrx_set_xtile": ([vp, i32, i32], st),
+        "corrx_get_xtile": ([vp, ctypes.POINTER(i32), ctypes.POINTER(i32)], st),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -418,6 +420,16 @@ class Engine:
         p = ctypes.c_void_p()
         check(self._lib.corrx_gains(self.h, ctypes.byref(p)), "corrx_gains")
         return p.value
+
+    def xtile(self):
+        """(channels per MFMA X-engine workgroup, channel tile of the spectrum order)."""
+        c = ctypes.c_int32()
+        t = ctypes.c_int32()
+        check(self._lib.corrx_get_xtile(self.h, ctypes.byref(c), ctypes.byref(t)), "corrx_get_xtile")
+        return c.value, t.value
+
+    def set_xtile(self, channels, spectra_tile):
+        check(self._lib.corrx_set_xtile(self.h, channels, spectra_tile), "corrx_set_xtile")
 
 
 def scale(d_acc, n_complex, frames, stream=None):

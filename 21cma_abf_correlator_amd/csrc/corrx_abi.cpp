@@ -93,3 +93,16 @@ extern "C" corr_status corrx_spectra(void *engine, void **d_spectra, size_t *sta
     *station_stride_elems = as_engine(engine)->spectra_station_stride();
     return CORR_OK;
 }
+
+extern "C" corr_status corrx_set_xtile(void *engine, int32_t channels, int32_t spectra_tile) {
+    if (engine == nullptr) return fail(CORR_ERR_ARG, "engine is NULL");
+    return as_engine(engine)->set_xtile(channels, spectra_tile);
+}
+
+extern "C" corr_status corrx_get_xtile(void *engine, int32_t *channels, int32_t *spectra_tile) {
+    if (engine == nullptr) return fail(CORR_ERR_ARG, "engine is NULL");
+    if (channels == nullptr || spectra_tile == nullptr) return fail(CORR_ERR_ARG, "output pointer is NULL");
+    *channels = as_engine(engine)->xtile();
+    *spectra_tile = as_engine(engine)->spec_tile();
+    return CORR_OK;
+}

@@ -5,11 +5,15 @@
 #define CORR_HD __host__ __device__
 #endif
 #include "xengine_body.hpp"
+#include "xmfma_body.hpp"
 
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <new>
+#include <tuple>
 #include <vector>
 
 namespace corr {
@@ -144,14 +148,128 @@ corr_status Engine::init() {
         std::memcpy(&one_bits, &one, 4);
         if ((st = hip_check(launch_fill(d_gain_, g, one_bits, nullptr), "fill_kernel(gains)")) != CORR_OK)
             return st;
-        if ((st = hip_check(xengine_mfma_prepare((int)shape_.form, N), "hipFuncSetAttribute(xengine_mfma)")) !=
-            CORR_OK)
-            return st;
+        for (int ch : {16, 32})
+            if (xmfma::tile_fits(N, ch) && nf % ch == 0 &&
+                (st = hip_check(xengine_mfma_prepare((int)shape_.form, N, ch),
+                                "hipFuncSetAttribute(xengine_mfma)")) != CORR_OK)
+                return st;
     }
 
     if ((st = hip_check(launch_twiddles(d_tw_, d_wl_, nf, nullptr), "twiddle_kernel")) != CORR_OK)
         return st;
-    return hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize(twiddles)");
+    if ((st = hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize(twiddles)")) != CORR_OK) return st;
+    return shape_.form == Form::kFP32 ? CORR_OK : tune_xtile();
+}
+
+namespace {
+struct TileKey {
+    int form, n_station, n_freq, device;
+    bool operator<(const TileKey &o) const {
+        return std::tie(form, n_station, n_freq, device) < std::tie(o.form, o.n_station, o.n_freq, o.device);
+    }
+};
+std::mutex g_tile_mu;
+std::map<TileKey, std::pair<int, int>> g_tile_cache;  // (channels per workgroup, spectrum tile)
+}  // namespace
+
+// The tile autotuner. Two choices interact: the channel width of the
+// X-engine's workgroups (16, or 32 up to 24 inputs) and the channel tile of
+// the spectrum order (that width, whose runs the X-engine reads contiguously,
+// or n_freq, the plain order the F-engine writes fastest). Every combination
+// that fits the shape is timed on the engine's own buffers (F-engine over
+// min(chunk_frames, 512) frames of zeros, X-engine over them: the call size,
+// where the steady state of both kernels shows; one warm-up and three timed
+// rounds); the fastest wins. The choice is cached per (form, inputs, n_freq,
+// device).
+corr_status Engine::tune_xtile() {
+    const int N = shape_.n_station, nf = shape_.n_freq;
+    std::vector<std::pair<int, int>> cand;
+    for (int ch : {16, 32})
+        if (xmfma::tile_fits(N, ch) && nf % ch == 0) {
+            cand.emplace_back(ch, ch);
+            cand.emplace_back(ch, nf);
+        }
+    if (cand.empty()) return failf(CORR_ERR_NOT_IMPL, "xengine: no MFMA tile fits %d inputs", N);
+    x_tile_ = cand[0].first;
+    spec_tile_ = cand[0].second;
+    int dev = 0;
+    corr_status st;
+    if ((st = hip_check(hipGetDevice(&dev), "hipGetDevice")) != CORR_OK) return st;
+    const TileKey key{(int)shape_.form, N, nf, dev};
+    {
+        std::lock_guard<std::mutex> lk(g_tile_mu);
+        auto it = g_tile_cache.find(key);
+        if (it != g_tile_cache.end()) {
+            x_tile_ = it->second.first;
+            spec_tile_ = it->second.second;
+            return CORR_OK;
+        }
+    }
+    const int frames = shape_.chunk_frames < 512 ? shape_.chunk_frames : 512;
+    const size_t in_pitch = (size_t)frames * 2u * (size_t)nf;
+    const size_t integ_bytes = (size_t)n_baseline_ * (size_t)nf * integrator_elem_bytes();
+    void *integ = nullptr, *in = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    float best_ms = 0.0f;
+    std::pair<int, int> best = cand[0];
+    if ((st = hip_check(hipMalloc(&integ, integ_bytes), "hipMalloc(tuning integrator)")) == CORR_OK &&
+        (st = hip_check(hipMalloc(&in, in_pitch * (size_t)N), "hipMalloc(tuning input)")) == CORR_OK &&
+        (st = hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate")) == CORR_OK &&
+        (st = hip_check(hipEventCreate(&e0), "hipEventCreate")) == CORR_OK &&
+        (st = hip_check(hipEventCreate(&e1), "hipEventCreate")) == CORR_OK &&
+        (st = hip_check(launch_fill(in, in_pitch * (size_t)N / 4, 0u, s), "fill_kernel(input)")) == CORR_OK &&
+        (st = hip_check(launch_fill(integ, integ_bytes / 4, 0u, s), "fill_kernel(integrator)")) == CORR_OK) {
+        auto round = [&](const std::pair<int, int> &c) {
+            corr_status r = hip_check(launch_fengine_form((int)shape_.form, log2_nf_, static_cast<const int8_t *>(in),
+                                                          in_pitch, nullptr, 0, d_tw_, d_wl_, d_gain_, d_spec_,
+                                                          spectra_station_stride(), c.second, frames, N, s),
+                                      "fengine_form_kernel");
+            if (r != CORR_OK) return r;
+            return hip_check(launch_xengine_mfma((int)shape_.form, N, c.first, d_spec_, spectra_station_stride(),
+                                                 nf, c.second, frames, integ, s),
+                             "xengine_mfma_kernel");
+        };
+        for (const auto &c : cand) {
+            if ((st = round(c)) != CORR_OK || (st = hip_check(hipEventRecord(e0, s), "hipEventRecord")) != CORR_OK)
+                break;
+            for (int r = 0; r < 3 && st == CORR_OK; ++r) st = round(c);
+            float ms = 0.0f;
+            if (st != CORR_OK || (st = hip_check(hipEventRecord(e1, s), "hipEventRecord")) != CORR_OK ||
+                (st = hip_check(hipEventSynchronize(e1), "hipEventSynchronize")) != CORR_OK ||
+                (st = hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime")) != CORR_OK)
+                break;
+            if (c == cand[0] || ms < best_ms) {
+                best_ms = ms;
+                best = c;
+            }
+        }
+    }
+    if (s) hipStreamSynchronize(s);
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (s) hipStreamDestroy(s);
+    hipFree(integ);
+    hipFree(in);
+    if (st != CORR_OK) return st;
+    x_tile_ = best.first;
+    spec_tile_ = best.second;
+    std::lock_guard<std::mutex> lk(g_tile_mu);
+    g_tile_cache[key] = best;
+    return CORR_OK;
+}
+
+corr_status Engine::set_xtile(int channels, int spec_tile) {
+    if (shape_.form == Form::kFP32) return fail(CORR_ERR_ARG, "set_xtile: the FP32 form has no MFMA tile");
+    if (!xmfma::tile_fits(shape_.n_station, channels) || shape_.n_freq % channels != 0)
+        return failf(CORR_ERR_ARG, "set_xtile: %d channels per workgroup do not fit %d inputs / n_freq %d",
+                     channels, shape_.n_station, shape_.n_freq);
+    if (spec_tile != channels && spec_tile != shape_.n_freq)
+        return failf(CORR_ERR_ARG, "set_xtile: the spectrum tile must be %d or n_freq (%d), not %d", channels,
+                     shape_.n_freq, spec_tile);
+    x_tile_ = channels;
+    spec_tile_ = spec_tile;
+    return CORR_OK;
 }
 
 Engine::~Engine() {
@@ -211,7 +329,7 @@ corr_status Engine::fengine(const int8_t *d_in, size_t station_pitch, const uint
                          "fengine_fp32_kernel");
     return hip_check(launch_fengine_form((int)shape_.form, log2_nf_, d_in, station_pitch, valid,
                                          valid_stride, d_tw_, d_wl_, d_gain_, d_spec_,
-                                         spectra_station_stride(), frames, shape_.n_station, s),
+                                         spectra_station_stride(), spec_tile_, frames, shape_.n_station, s),
                      "fengine_form_kernel");
 }
 
@@ -220,8 +338,8 @@ corr_status Engine::xengine(void *integ, int frames, hipStream_t s) {
     if (integ == nullptr) return fail(CORR_ERR_ARG, "xengine: integrator is NULL");
     if (frames < 0 || frames > shape_.chunk_frames)
         return failf(CORR_ERR_ARG, "xengine: frames=%d outside [0, %d]", frames, shape_.chunk_frames);
-    return hip_check(launch_xengine_mfma((int)shape_.form, shape_.n_station, d_spec_,
-                                         spectra_station_stride(), shape_.n_freq, frames, integ, s),
+    return hip_check(launch_xengine_mfma((int)shape_.form, shape_.n_station, x_tile_, d_spec_,
+                                         spectra_station_stride(), shape_.n_freq, spec_tile_, frames, integ, s),
                      "xengine_mfma_kernel");
 }
 

@@ -98,11 +98,21 @@ class Engine {
     corr_status set_gains(const float *d_gains, hipStream_t s);
     const float *gains() const { return d_gain_; }
 
+    /// FP16/int8 forms: channels per X-engine workgroup (16 or 32) and the
+    /// channel tile of the spectrum buffer's order (the workgroup width or
+    /// n_freq, see forms::spec_index). Chosen at creation by timing F- and
+    /// X-engine with every combination that fits (once per process and
+    /// shape); set_xtile overrides it (before the next fengine call).
+    int xtile() const { return x_tile_; }
+    int spec_tile() const { return spec_tile_; }
+    corr_status set_xtile(int channels, int spec_tile);
+
     /// acc *= 1.0f/(float)frames over `n` complex values.
     static corr_status scale(float2 *acc, size_t n, uint64_t frames, hipStream_t s);
 
-    /// Spectrum buffer: [station][chunk_frames][n_freq] of the form's element
-    /// (complex64, packed FP16 pair in 32 bits, packed int8 pair in 16 bits).
+    /// Spectrum buffer: FP32 [station][chunk_frames][n_freq] complex64; FP16 /
+    /// int8 [station][n_freq/spec_tile][chunk_frames][spec_tile] of the packed
+    /// pair (forms::spec_index).
     void *spectra() const { return d_spec_; }
     size_t spectra_station_stride() const { return (size_t)shape_.chunk_frames * shape_.n_freq; }
     size_t spectra_elem_bytes() const;
@@ -119,6 +129,9 @@ class Engine {
     int log2_nf_ = 0;
 
     int cal_frames_ = 1;        ///< frames per calibration batch (FP32 spectra in d_spec_)
+    int x_tile_ = 16;           ///< channels per MFMA X-engine workgroup
+    int spec_tile_ = 16;        ///< channel tile of the spectrum order (x_tile_ or n_freq)
+    corr_status tune_xtile();
 
     float2 *d_tw_ = nullptr;    ///< exp(-2 pi i m / n_freq), m < n_freq
     float2 *d_wl_ = nullptr;    ///< exp(-2 pi i k / (2 n_freq)), k < n_freq
@@ -154,7 +167,7 @@ hipError_t launch_scale(float2 *acc, size_t n, float s, hipStream_t st);
 hipError_t launch_fengine_form(int form, int log2_nf, const int8_t *in, size_t station_pitch,
                                const int32_t *valid, size_t valid_stride, const float2 *tw,
                                const float2 *wl, const float *gain, void *out,
-                               size_t out_station_stride, int frames, int n_station,
+                               size_t out_station_stride, int tile, int frames, int n_station,
                                hipStream_t s);
 hipError_t launch_power(int nf, int frames, int n_station, const float2 *spec,
                         size_t spec_station_stride, const int32_t *valid, size_t valid_stride,
@@ -162,9 +175,9 @@ hipError_t launch_power(int nf, int frames, int n_station, const float2 *spec,
 hipError_t launch_gains(int form, int nf, int n_station, const float *power, const int32_t *count,
                         float *gain, hipStream_t s);
 hipError_t launch_fill(void *p, size_t n_words, uint32_t v, hipStream_t s);
-hipError_t xengine_mfma_prepare(int form, int n_station);
-hipError_t launch_xengine_mfma(int form, int n_station, const void *spectra, size_t ys, int nf,
-                               int frames, void *integ, hipStream_t s);
+hipError_t xengine_mfma_prepare(int form, int n_station, int ch);
+hipError_t launch_xengine_mfma(int form, int n_station, int ch, const void *spectra, size_t ys, int nf,
+                               int spec_tile, int frames, void *integ, hipStream_t s);
 hipError_t launch_dump(int form, const void *integ, int n_baseline, int nf, const int2 *pairs,
                        const float *gain, uint64_t frames, float2 *out, hipStream_t s);
 

@@ -94,12 +94,12 @@ fengine_form_kernel(const int8_t *__restrict__ in, size_t station_pitch,
                     const int32_t *__restrict__ valid, size_t valid_stride,
                     const float2 *__restrict__ tw, const float2 *__restrict__ wl,
                     const float *__restrict__ gain, void *__restrict__ out,
-                    size_t out_station_stride) {
+                    size_t out_station_stride, int tile_shift) {
     __shared__ float2 lds[fft::fengine_lds_elems<LOG2N>()];
     DeviceSync sync;
     fft::fengine_form_block<LOG2N, FORM>((int)threadIdx.x, (int)blockIdx.x, (int)blockIdx.y, lds,
                                          sync, in, station_pitch, valid, valid_stride, tw, wl, gain,
-                                         out, out_station_stride);
+                                         out, out_station_stride, tile_shift);
 }
 
 // Validity bits -> int32 per (station, frame); grid (ceil(frames/256), stations).
@@ -154,20 +154,21 @@ __global__ void __launch_bounds__(kScaleBlock) scale_kernel(float2 *acc, size_t 
 
 // ---------------------------------------------------------------------------
 // MFMA X-engine (FP16 / int8 forms)
-// grid (n_freq/16), block 512; dynamic LDS xmfma::lds_bytes(n_station)
+// grid (n_freq/CH), block 32*CH; dynamic LDS xmfma::lds_bytes(n_station, CH).
+// Up to 24 inputs (3 row tiles) a CU holds 16 waves of it (two 16-channel or
+// one 32-channel workgroup), which keeps each lane to 128 registers.
 // ---------------------------------------------------------------------------
-// Up to 24 inputs (3 row tiles) two workgroups share a CU (LDS <= 62 KB each),
-// which holds each lane to 128 registers.
-template <int FORM, int NRT>
-__global__ void __launch_bounds__(xmfma::kThreads, NRT <= 3 ? 2 : 1)
-xengine_mfma_kernel(const void *__restrict__ spectra, size_t ys, int nf, int frames, int n_station,
-                    int n_tiles, void *__restrict__ integ) {
+template <int FORM, int NRT, int CH>
+__global__ void __launch_bounds__(xmfma::Tile<CH>::kThreads)
+__attribute__((amdgpu_waves_per_eu(NRT <= 3 ? 4 : 1)))
+xengine_mfma_kernel(const void *__restrict__ spectra, size_t ys, int nf, int spec_tile, int frames,
+                    int n_station, int n_tiles, void *__restrict__ integ) {
     extern __shared__ uint4 lds_words[];
     DeviceSync sync;
     MfmaOps ops;
-    xmfma::xengine_mfma_block<FORM, NRT>((int)threadIdx.x, (int)blockIdx.x, n_tiles,
-                                         reinterpret_cast<uint8_t *>(lds_words), sync, ops, spectra,
-                                         ys, nf, frames, n_station, integ);
+    xmfma::xengine_mfma_block<FORM, NRT, CH>((int)threadIdx.x, (int)blockIdx.x, n_tiles,
+                                             reinterpret_cast<uint8_t *>(lds_words), sync, ops,
+                                             spectra, ys, nf, spec_tile, frames, n_station, integ);
 }
 
 // Calibration power sum and gains; grid (ceil(nf/256), stations), block 256.
@@ -231,11 +232,11 @@ template <int LOG2N, int FORM>
 hipError_t launch_fengine_form_t(const int8_t *in, size_t station_pitch, const int32_t *valid,
                                  size_t valid_stride, const float2 *tw, const float2 *wl,
                                  const float *gain, void *out, size_t out_station_stride,
-                                 int frames, int n_station, hipStream_t s) {
+                                 int tile_shift, int frames, int n_station, hipStream_t s) {
     const dim3 grid((unsigned)frames, (unsigned)n_station);
     const dim3 block((1u << LOG2N) / 16u);
     hipLaunchKernelGGL((fengine_form_kernel<LOG2N, FORM>), grid, block, 0, s, in, station_pitch,
-                       valid, valid_stride, tw, wl, gain, out, out_station_stride);
+                       valid, valid_stride, tw, wl, gain, out, out_station_stride, tile_shift);
     return hipGetLastError();
 }
 
@@ -243,14 +244,14 @@ template <int FORM>
 hipError_t launch_fengine_form_f(int log2_nf, const int8_t *in, size_t station_pitch,
                                  const int32_t *valid, size_t valid_stride, const float2 *tw,
                                  const float2 *wl, const float *gain, void *out,
-                                 size_t out_station_stride, int frames, int n_station,
-                                 hipStream_t s) {
+                                 size_t out_station_stride, int tile_shift, int frames,
+                                 int n_station, hipStream_t s) {
     switch (log2_nf) {
 #define CORR_FFORM_CASE(L)                                                                       \
     case L:                                                                                      \
         return launch_fengine_form_t<L, FORM>(in, station_pitch, valid, valid_stride, tw, wl,   \
-                                              gain, out, out_station_stride, frames, n_station, \
-                                              s);
+                                              gain, out, out_station_stride, tile_shift, frames, \
+                                              n_station, s);
         CORR_FFORM_CASE(9)
         CORR_FFORM_CASE(10)
         CORR_FFORM_CASE(11)
@@ -263,42 +264,55 @@ hipError_t launch_fengine_form_f(int log2_nf, const int8_t *in, size_t station_p
     }
 }
 
-template <int FORM>
+template <int FORM, int CH>
 const void *xengine_mfma_fn(int nrt) {
     switch (nrt) {
-    case 1: return reinterpret_cast<const void *>(&xengine_mfma_kernel<FORM, 1>);
-    case 2: return reinterpret_cast<const void *>(&xengine_mfma_kernel<FORM, 2>);
-    case 3: return reinterpret_cast<const void *>(&xengine_mfma_kernel<FORM, 3>);
-    case 4: return reinterpret_cast<const void *>(&xengine_mfma_kernel<FORM, 4>);
-    case 5: return reinterpret_cast<const void *>(&xengine_mfma_kernel<FORM, 5>);
-    case 6: return reinterpret_cast<const void *>(&xengine_mfma_kernel<FORM, 6>);
-    default: return nullptr;
+    case 1: return reinterpret_cast<const void *>(&xengine_mfma_kernel<FORM, 1, CH>);
+    case 2: return reinterpret_cast<const void *>(&xengine_mfma_kernel<FORM, 2, CH>);
+    case 3: return reinterpret_cast<const void *>(&xengine_mfma_kernel<FORM, 3, CH>);
+    default: break;
     }
+    if constexpr (CH == 16) {
+        switch (nrt) {
+        case 4: return reinterpret_cast<const void *>(&xengine_mfma_kernel<FORM, 4, CH>);
+        case 5: return reinterpret_cast<const void *>(&xengine_mfma_kernel<FORM, 5, CH>);
+        case 6: return reinterpret_cast<const void *>(&xengine_mfma_kernel<FORM, 6, CH>);
+        default: break;
+        }
+    }
+    return nullptr;
 }
 
-template <int FORM>
-hipError_t launch_xengine_mfma_f(int n_station, const void *spectra, size_t ys, int nf, int frames,
-                                 void *integ, hipStream_t s) {
+template <int FORM, int CH>
+hipError_t launch_xengine_mfma_f(int n_station, const void *spectra, size_t ys, int nf, int spec_tile,
+                                 int frames, void *integ, hipStream_t s) {
     const int nrt = xmfma::row_tiles(n_station);
-    const int n_tiles = nf / xmfma::kChannels;
-    const size_t shm = xmfma::lds_bytes(n_station);
-    const dim3 grid((unsigned)n_tiles), block((unsigned)xmfma::kThreads);
-    switch (nrt) {
-#define CORR_XMFMA_CASE(R)                                                                          \
-    case R:                                                                                         \
-        hipLaunchKernelGGL((xengine_mfma_kernel<FORM, R>), grid, block, shm, s, spectra, ys, nf,   \
-                           frames, n_station, n_tiles, integ);                                      \
+    const int n_tiles = nf / CH;
+    const size_t shm = xmfma::lds_bytes(n_station, CH);
+    const dim3 grid((unsigned)n_tiles), block((unsigned)xmfma::Tile<CH>::kThreads);
+#define CORR_XMFMA_CASE(R)                                                                           \
+    case R:                                                                                          \
+        hipLaunchKernelGGL((xengine_mfma_kernel<FORM, R, CH>), grid, block, shm, s, spectra, ys, nf, \
+                           spec_tile, frames, n_station, n_tiles, integ);                            \
         return hipGetLastError();
+    switch (nrt) {
         CORR_XMFMA_CASE(1)
         CORR_XMFMA_CASE(2)
         CORR_XMFMA_CASE(3)
-        CORR_XMFMA_CASE(4)
-        CORR_XMFMA_CASE(5)
-        CORR_XMFMA_CASE(6)
-#undef CORR_XMFMA_CASE
     default:
-        return hipErrorInvalidValue;
+        break;
     }
+    if constexpr (CH == 16) {
+        switch (nrt) {
+            CORR_XMFMA_CASE(4)
+            CORR_XMFMA_CASE(5)
+            CORR_XMFMA_CASE(6)
+        default:
+            break;
+        }
+    }
+#undef CORR_XMFMA_CASE
+    return hipErrorInvalidValue;
 }
 
 template <int LOG2N>
@@ -379,15 +393,17 @@ hipError_t launch_xengine_fp32(const float2 *y, size_t y_station_stride, int n_f
 hipError_t launch_fengine_form(int form, int log2_nf, const int8_t *in, size_t station_pitch,
                                const int32_t *valid, size_t valid_stride, const float2 *tw,
                                const float2 *wl, const float *gain, void *out,
-                               size_t out_station_stride, int frames, int n_station,
+                               size_t out_station_stride, int tile, int frames, int n_station,
                                hipStream_t s) {
     if (frames <= 0 || n_station <= 0) return hipSuccess;
+    if (tile != 16 && tile != 32 && tile != (1 << log2_nf)) return hipErrorInvalidValue;
+    const int shift = tile == 16 ? 4 : tile == 32 ? 5 : log2_nf;  // a tile of n_freq: [station][frame][channel]
     if (form == 1)
         return launch_fengine_form_f<1>(log2_nf, in, station_pitch, valid, valid_stride, tw, wl, gain,
-                                        out, out_station_stride, frames, n_station, s);
+                                        out, out_station_stride, shift, frames, n_station, s);
     if (form == 2)
         return launch_fengine_form_f<2>(log2_nf, in, station_pitch, valid, valid_stride, tw, wl, gain,
-                                        out, out_station_stride, frames, n_station, s);
+                                        out, out_station_stride, shift, frames, n_station, s);
     return hipErrorInvalidValue;
 }
 
@@ -418,20 +434,28 @@ hipError_t launch_fill(void *p, size_t n_words, uint32_t v, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t xengine_mfma_prepare(int form, int n_station) {
+hipError_t xengine_mfma_prepare(int form, int n_station, int ch) {
+    if (!xmfma::tile_fits(n_station, ch)) return hipErrorInvalidValue;
     const int nrt = xmfma::row_tiles(n_station);
-    const void *fn = form == 1 ? xengine_mfma_fn<1>(nrt) : form == 2 ? xengine_mfma_fn<2>(nrt) : nullptr;
+    const void *fn = nullptr;
+    if (form == 1) fn = ch == 16 ? xengine_mfma_fn<1, 16>(nrt) : xengine_mfma_fn<1, 32>(nrt);
+    if (form == 2) fn = ch == 16 ? xengine_mfma_fn<2, 16>(nrt) : xengine_mfma_fn<2, 32>(nrt);
     if (fn == nullptr) return hipErrorInvalidValue;
     return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)xmfma::lds_bytes(n_station));
+                               (int)xmfma::lds_bytes(n_station, ch));
 }
 
-hipError_t launch_xengine_mfma(int form, int n_station, const void *spectra, size_t ys, int nf,
-                               int frames, void *integ, hipStream_t s) {
+hipError_t launch_xengine_mfma(int form, int n_station, int ch, const void *spectra, size_t ys, int nf,
+                               int spec_tile, int frames, void *integ, hipStream_t s) {
     if (frames <= 0) return hipSuccess;
-    if (nf % xmfma::kChannels != 0) return hipErrorInvalidValue;
-    if (form == 1) return launch_xengine_mfma_f<1>(n_station, spectra, ys, nf, frames, integ, s);
-    if (form == 2) return launch_xengine_mfma_f<2>(n_station, spectra, ys, nf, frames, integ, s);
+    if (!xmfma::tile_fits(n_station, ch) || nf % ch != 0) return hipErrorInvalidValue;
+    if (spec_tile != ch && spec_tile != nf) return hipErrorInvalidValue;
+    if (form == 1)
+        return ch == 16 ? launch_xengine_mfma_f<1, 16>(n_station, spectra, ys, nf, spec_tile, frames, integ, s)
+                        : launch_xengine_mfma_f<1, 32>(n_station, spectra, ys, nf, spec_tile, frames, integ, s);
+    if (form == 2)
+        return ch == 16 ? launch_xengine_mfma_f<2, 16>(n_station, spectra, ys, nf, spec_tile, frames, integ, s)
+                        : launch_xengine_mfma_f<2, 32>(n_station, spectra, ys, nf, spec_tile, frames, integ, s);
     return hipErrorInvalidValue;
 }
 

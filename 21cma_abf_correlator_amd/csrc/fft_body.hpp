@@ -391,23 +391,29 @@ CORR_HD CORR_FI void fengine_fp32_block(int j, int f, int st, float2 *lds, Sync 
 
 // Whole F-engine workgroup for the FP16 (FORM 1) or int8 (FORM 2) form: bin k
 // of station st is scaled by gain[st*N + k] and converted (forms.hpp); output
-// [station][frame][N] of packed 32-bit (FP16) or 16-bit (int8) complex.
+// packed 32-bit (FP16) or 16-bit (int8) complex in the tile-major order of
+// forms::spec_index with tiles of 2^tile_shift channels, out_station_stride =
+// N * frame capacity.
 template <int LOG2N, int FORM, class Sync>
 CORR_HD CORR_FI void fengine_form_block(int j, int f, int st, float2 *lds, Sync &sync, const int8_t *in,
                                        size_t station_pitch, const int32_t *valid, size_t valid_stride,
                                        const float2 *tw, const float2 *wl, const float *gain,
-                                       void *out, size_t out_station_stride) {
+                                       void *out, size_t out_station_stride, int tile_shift) {
     constexpr int N = 1 << LOG2N;
     const float *g = gain + (size_t)st * N;
-    const size_t at = (size_t)st * out_station_stride + (size_t)f * (size_t)N;
+    // bin k -> tile k >> shift at (tile * fcap + f) * T + k % T from the station's base
+    const unsigned tmask = (1u << tile_shift) - 1u;
+    const unsigned tile_stride = (unsigned)(out_station_stride / N) << tile_shift;  // fcap * T
+    const size_t at = (size_t)st * out_station_stride + ((size_t)f << tile_shift);
+    auto off = [&](int k) { return ((unsigned)k >> tile_shift) * tile_stride + ((unsigned)k & tmask); };
     if constexpr (FORM == 1) {
         uint32_t *dst = static_cast<uint32_t *>(out) + at;
         fengine_frame<LOG2N>(j, f, st, lds, sync, in, station_pitch, valid, valid_stride, tw, wl,
-                             [&](int k, float2 x) { dst[(unsigned)k] = forms::pack_fp16(x, g[(unsigned)k]); });
+                             [&](int k, float2 x) { dst[off(k)] = forms::pack_fp16(x, g[(unsigned)k]); });
     } else {
         uint16_t *dst = static_cast<uint16_t *>(out) + at;
         fengine_frame<LOG2N>(j, f, st, lds, sync, in, station_pitch, valid, valid_stride, tw, wl,
-                             [&](int k, float2 x) { dst[(unsigned)k] = forms::pack_int8(x, g[(unsigned)k]); });
+                             [&](int k, float2 x) { dst[off(k)] = forms::pack_int8(x, g[(unsigned)k]); });
     }
 }
 

@@ -101,5 +101,18 @@ CORR_HD inline uint16_t pack_int8(float2 y, float g) {
     return (uint16_t)((uint32_t)(quant_int8(y.x * g) & 0xFF) | ((uint32_t)(quant_int8(y.y * g) & 0xFF) << 8));
 }
 
+// Spectrum buffer of the FP16 and int8 forms: channels in tiles of `tile` (16
+// or 32, the X-engine's channels per workgroup), each tile's frames
+// contiguous, [station][n_freq/tile][fcap frames][tile channels] (fcap = the
+// buffer's frame capacity). The X-engine then reads one tile's frames as a
+// contiguous run (the [station][frame][channel] order would have it read
+// pieces of a row 64 KB apart, which halves the HBM rate), and the F-engine
+// writes pieces of tile*element bytes (whole 128-byte lines for 32 FP16
+// channels).
+CORR_HD inline size_t spec_index(int st, int f, int c, int nf, int fcap, int tile) {
+    return (((size_t)st * (size_t)(nf / tile) + (size_t)(c / tile)) * (size_t)fcap + (size_t)f) * (size_t)tile +
+           (size_t)(c % tile);
+}
+
 }  // namespace forms
 }  // namespace corr

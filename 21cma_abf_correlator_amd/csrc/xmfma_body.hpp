@@ -14,8 +14,8 @@
 // exact in int32; the chunk's sum is added into the integrator once per call,
 // with Neumaier-compensated FP32 (FP16 form) or exactly in int32 (int8 form).
 //
-// Workgroup: 16 channels, 8 waves, 2 channels per wave. One step stages 32
-// (FP16) or 64 (int8) frames of all inputs for the 16 channels in LDS as
+// Workgroup: CH = 16 or 32 channels, CH/2 waves, 2 channels per wave. One step
+// stages 32 (FP16) or 64 (int8) frames of all inputs for the CH channels in LDS as
 //   lds[channel][row][frame]  (row pitch 80 B = 64 data + 16 pad,
 //                              channel pitch rows_padded*80 + 16 B)
 // so that a lane's MFMA operand (16 B: 8 halves / 16 int8 of one row) is one
@@ -35,15 +35,26 @@
 #include <cstddef>
 #include <cstdint>
 
+#include "forms.hpp"
+
 namespace corr {
 namespace xmfma {
 
-constexpr int kChannels = 16;       ///< channels per workgroup
-constexpr int kWaves = 8;           ///< waves per workgroup
-constexpr int kThreads = 64 * kWaves;
-constexpr int kChanPerWave = kChannels / kWaves;
 constexpr int kRowBytes = 80;       ///< 64 bytes of one row per step + 16 pad
 constexpr int kMaxRowTiles = 6;     ///< 2*n_station <= 96
+constexpr int kChanPerWave = 2;
+constexpr int kMaxThreads = 1024;
+
+/// Workgroup shape for CH channels per workgroup.
+template <int CH>
+struct Tile {
+    static_assert(CH == 16 || CH == 32, "16 or 32 channels per workgroup");
+    static constexpr int kChannels = CH;
+    static constexpr int kWaves = CH / kChanPerWave;
+    static constexpr int kThreads = 64 * kWaves;
+    static constexpr int kQuads = CH / 4;       ///< channel quads per row
+    static constexpr int kOutPitch = CH + 1;    ///< results per baseline in LDS (one spare: banks)
+};
 
 /// One lane's MFMA operand: 16 bytes.
 struct Frag {
@@ -78,16 +89,22 @@ CORR_HD inline int rows_padded(int n_station) { return ((2 * n_station + 15) / 1
 CORR_HD inline int row_tiles(int n_station) { return rows_padded(n_station) / 16; }
 CORR_HD inline size_t chan_pitch(int n_station) { return (size_t)rows_padded(n_station) * kRowBytes + 16; }
 
-/// Results of one call in LDS: [baseline][kOutPitch] complex, the pitch one
-/// more than the channels so that one channel's baselines spread over banks.
-constexpr int kOutPitch = kChannels + 1;
-
-/// Dynamic LDS bytes: the staging tile, reused for the results.
-CORR_HD inline size_t lds_bytes(int n_station) {
-    const size_t stage = (size_t)kChannels * chan_pitch(n_station);
+/// Dynamic LDS bytes: the staging tile of ch channels, reused for the results
+/// of one call, [baseline][ch + 1] complex (the pitch one more than the
+/// channels, so that one channel's baselines spread over banks).
+CORR_HD inline size_t lds_bytes(int n_station, int ch) {
+    const size_t stage = (size_t)ch * chan_pitch(n_station);
     const size_t nb = (size_t)n_station * (size_t)(n_station + 1) / 2;
-    const size_t out = nb * kOutPitch * 8;
+    const size_t out = nb * (size_t)(ch + 1) * 8;
     return stage > out ? stage : out;
+}
+
+/// Largest LDS of one workgroup (gfx950).
+constexpr size_t kLdsLimit = 160 * 1024;
+
+/// A tile of ch channels is available for n_station inputs.
+CORR_HD inline bool tile_fits(int n_station, int ch) {
+    return (ch == 16 || (ch == 32 && row_tiles(n_station) <= 3)) && lds_bytes(n_station, ch) <= kLdsLimit;
 }
 
 /// Workgroup -> channel tile. Consecutive workgroups are spread over the eight
@@ -118,112 +135,134 @@ struct Stage {
     uint32_t w[32];
 };
 
-template <int FORM>
-CORR_HD inline void load_item(int item, const void *spectra, size_t ys, int nf, int frames, int f0, int c0,
-                              Stage &st);
-
-template <>
-CORR_HD inline void load_item<1>(int item, const void *spectra, size_t ys, int nf, int frames, int f0, int c0,
-                                 Stage &st) {
-    const int q = item & 3, o = (item >> 2) & 3, s = item >> 4;
-    const uint32_t *y = static_cast<const uint32_t *>(spectra);
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int f = f0 + 8 * o + i;
-        if (f < frames) {
-            const uint32_t *src = y + (size_t)s * ys + (size_t)f * (size_t)nf + (size_t)(c0 + 4 * q);
-            const uint4 v = *reinterpret_cast<const uint4 *>(src);
-            st.w[4 * i + 0] = v.x;
-            st.w[4 * i + 1] = v.y;
-            st.w[4 * i + 2] = v.z;
-            st.w[4 * i + 3] = v.w;
-        } else {
-            st.w[4 * i + 0] = st.w[4 * i + 1] = st.w[4 * i + 2] = st.w[4 * i + 3] = 0u;
-        }
-    }
+// Item = (input s, frame group o of 4, channel quad q of CH/4), q fastest, so
+// that a wave's lanes read whole rows.
+template <int CH>
+CORR_HD inline void item_coords(int item, int &q, int &o, int &s) {
+    constexpr int QD = CH / 4;
+    q = item % QD;
+    o = (item / QD) & 3;
+    s = item / (4 * QD);
 }
 
-template <>
-CORR_HD inline void load_item<2>(int item, const void *spectra, size_t ys, int nf, int frames, int f0, int c0,
-                                 Stage &st) {
-    const int q = item & 3, o = (item >> 2) & 3, s = item >> 4;
-    const uint16_t *y = static_cast<const uint16_t *>(spectra);
-#pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        const int f = f0 + 16 * o + i;
-        if (f < frames) {
-            const uint16_t *src = y + (size_t)s * ys + (size_t)f * (size_t)nf + (size_t)(c0 + 4 * q);
-            const uint2 v = *reinterpret_cast<const uint2 *>(src);
-            st.w[2 * i + 0] = v.x;  // channels 0, 1 of the quad: re0 im0 re1 im1
-            st.w[2 * i + 1] = v.y;  // channels 2, 3
-        } else {
-            st.w[2 * i + 0] = st.w[2 * i + 1] = 0u;
-        }
-    }
-}
+template <int FORM, int CH>
+struct Staging;
 
-template <int FORM>
-CORR_HD inline void store_item(int item, uint8_t *lds, size_t cpitch, const Stage &st);
-
-template <>
-CORR_HD inline void store_item<1>(int item, uint8_t *lds, size_t cpitch, const Stage &st) {
-    const int q = item & 3, o = (item >> 2) & 3, s = item >> 4;
+template <int CH>
+struct Staging<1, CH> {  // FP16: 8 frames x 16 B (4 channels)
+    static CORR_HD inline void load(int item, const void *spectra, int nf, int fcap, int tile, int frames,
+                                    int f0, int c0, Stage &st) {
+        int q, o, s;
+        item_coords<CH>(item, q, o, s);
+        // frames f0 + 8o .. +7 of channels c0 + 4q .. +3: rows `tile` elements apart
+        const uint32_t *y = static_cast<const uint32_t *>(spectra) +
+                            forms::spec_index(s, f0 + 8 * o, c0 + 4 * q, nf, fcap, tile);
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
-        const uint32_t *w = st.w + m;  // w[4 i] = frame i's word of channel m
-        uint4 re, im;
-        re.x = (w[0] & 0xFFFFu) | (w[4] << 16);
-        re.y = (w[8] & 0xFFFFu) | (w[12] << 16);
-        re.z = (w[16] & 0xFFFFu) | (w[20] << 16);
-        re.w = (w[24] & 0xFFFFu) | (w[28] << 16);
-        im.x = (w[0] >> 16) | (w[4] & 0xFFFF0000u);
-        im.y = (w[8] >> 16) | (w[12] & 0xFFFF0000u);
-        im.z = (w[16] >> 16) | (w[20] & 0xFFFF0000u);
-        im.w = (w[24] >> 16) | (w[28] & 0xFFFF0000u);
-        uint8_t *row = lds + (size_t)(4 * q + m) * cpitch + (size_t)(2 * s) * kRowBytes + 16 * o;
-        *reinterpret_cast<uint4 *>(row) = re;
-        *reinterpret_cast<uint4 *>(row + kRowBytes) = im;
-    }
-}
-
-template <>
-CORR_HD inline void store_item<2>(int item, uint8_t *lds, size_t cpitch, const Stage &st) {
-    const int q = item & 3, o = (item >> 2) & 3, s = item >> 4;
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-#pragma unroll
-        for (int p = 0; p < 2; ++p) {
-            const int sh = 8 * (2 * (m & 1) + p);  // byte of (channel m, part p)
-            uint4 r;
-            uint32_t *rw = &r.x;
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                uint32_t acc = 0;
-#pragma unroll
-                for (int b = 0; b < 4; ++b) acc |= ((st.w[2 * (4 * k + b) + (m >> 1)] >> sh) & 0xFFu) << (8 * b);
-                rw[k] = acc;
+        for (int i = 0; i < 8; ++i) {
+            const int f = f0 + 8 * o + i;
+            if (f < frames) {
+                const uint32_t *src = y + (size_t)i * (size_t)tile;
+                const uint4 v = *reinterpret_cast<const uint4 *>(src);
+                st.w[4 * i + 0] = v.x;
+                st.w[4 * i + 1] = v.y;
+                st.w[4 * i + 2] = v.z;
+                st.w[4 * i + 3] = v.w;
+            } else {
+                st.w[4 * i + 0] = st.w[4 * i + 1] = st.w[4 * i + 2] = st.w[4 * i + 3] = 0u;
             }
-            uint8_t *row = lds + (size_t)(4 * q + m) * cpitch + (size_t)(2 * s + p) * kRowBytes + 16 * o;
-            *reinterpret_cast<uint4 *>(row) = r;
         }
     }
-}
 
-/// Whole X-engine workgroup (16 channels) for `frames` frames; NRT = row tiles.
+    static CORR_HD inline void store(int item, uint8_t *lds, size_t cpitch, const Stage &st) {
+        int q, o, s;
+        item_coords<CH>(item, q, o, s);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const uint32_t *w = st.w + m;  // w[4 i] = frame i's word of channel m
+            uint4 re, im;
+            re.x = (w[0] & 0xFFFFu) | (w[4] << 16);
+            re.y = (w[8] & 0xFFFFu) | (w[12] << 16);
+            re.z = (w[16] & 0xFFFFu) | (w[20] << 16);
+            re.w = (w[24] & 0xFFFFu) | (w[28] << 16);
+            im.x = (w[0] >> 16) | (w[4] & 0xFFFF0000u);
+            im.y = (w[8] >> 16) | (w[12] & 0xFFFF0000u);
+            im.z = (w[16] >> 16) | (w[20] & 0xFFFF0000u);
+            im.w = (w[24] >> 16) | (w[28] & 0xFFFF0000u);
+            uint8_t *row = lds + (size_t)(4 * q + m) * cpitch + (size_t)(2 * s) * kRowBytes + 16 * o;
+            *reinterpret_cast<uint4 *>(row) = re;
+            *reinterpret_cast<uint4 *>(row + kRowBytes) = im;
+        }
+    }
+};
+
+template <int CH>
+struct Staging<2, CH> {  // int8: 16 frames x 8 B (4 channels)
+    static CORR_HD inline void load(int item, const void *spectra, int nf, int fcap, int tile, int frames,
+                                    int f0, int c0, Stage &st) {
+        int q, o, s;
+        item_coords<CH>(item, q, o, s);
+        // frames f0 + 16o .. +15 of channels c0 + 4q .. +3: rows `tile` elements apart
+        const uint16_t *y = static_cast<const uint16_t *>(spectra) +
+                            forms::spec_index(s, f0 + 16 * o, c0 + 4 * q, nf, fcap, tile);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int f = f0 + 16 * o + i;
+            if (f < frames) {
+                const uint16_t *src = y + (size_t)i * (size_t)tile;
+                const uint2 v = *reinterpret_cast<const uint2 *>(src);
+                st.w[2 * i + 0] = v.x;  // channels 0, 1 of the quad: re0 im0 re1 im1
+                st.w[2 * i + 1] = v.y;  // channels 2, 3
+            } else {
+                st.w[2 * i + 0] = st.w[2 * i + 1] = 0u;
+            }
+        }
+    }
+
+    static CORR_HD inline void store(int item, uint8_t *lds, size_t cpitch, const Stage &st) {
+        int q, o, s;
+        item_coords<CH>(item, q, o, s);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                const int sh = 8 * (2 * (m & 1) + p);  // byte of (channel m, part p)
+                uint4 r;
+                uint32_t *rw = &r.x;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    uint32_t acc = 0;
+#pragma unroll
+                    for (int b = 0; b < 4; ++b)
+                        acc |= ((st.w[2 * (4 * k + b) + (m >> 1)] >> sh) & 0xFFu) << (8 * b);
+                    rw[k] = acc;
+                }
+                uint8_t *row = lds + (size_t)(4 * q + m) * cpitch + (size_t)(2 * s + p) * kRowBytes + 16 * o;
+                *reinterpret_cast<uint4 *>(row) = r;
+            }
+        }
+    }
+};
+
+/// Whole X-engine workgroup (CH channels) for `frames` frames; NRT = row tiles.
 /// `ops` provides mma(Frag, Frag, Acc) and xor1(value) as wave-wide operations;
 /// every lane of every wave calls them the same number of times.
-template <int FORM, int NRT, class Sync, class Ops>
+template <int FORM, int NRT, int CH, class Sync, class Ops>
 CORR_HD inline void xengine_mfma_block(int tid, int bx, int n_tiles, uint8_t *lds, Sync &sync, Ops &ops,
-                                       const void *spectra, size_t ys, int nf, int frames,
+                                       const void *spectra, size_t ys, int nf, int spec_tile, int frames,
                                        int n_station, void *integ) {
     using T = Traits<FORM>;
     using acc_t = typename T::acc_t;
+    using St = Staging<FORM, CH>;
+    constexpr int kChannels = Tile<CH>::kChannels;
+    constexpr int kThreads = Tile<CH>::kThreads;
+    constexpr int kOutPitch = Tile<CH>::kOutPitch;
     constexpr int NTP = NRT * (NRT + 1) / 2;
     const int lane = tid & 63, w = tid >> 6;
     const int c0 = channel_tile(bx, n_tiles) * kChannels;
     const int rp = NRT * 16;
     const size_t cpitch = (size_t)rp * kRowBytes + 16;
-    const int n_items = n_station * 16;
+    const int n_items = n_station * kChannels;  // inputs x 4 frame groups x CH/4 quads
+    const int fcap = (int)(ys / (size_t)nf);    // frame capacity of the spectrum buffer
 
     // Rows 2N .. rp-1 are never staged: zero them once.
     for (int i = tid; i < kChannels * (rp - 2 * n_station) * (kRowBytes / 16); i += kThreads) {
@@ -247,26 +286,26 @@ CORR_HD inline void xengine_mfma_block(int tid, int bx, int n_tiles, uint8_t *ld
     // the same registers, so they are in flight during the MFMA phase.
     // With two items per lane (more than 32 inputs) the registers of a second
     // staged step do not fit beside the accumulators: those load each step in turn.
-    constexpr int IPT = (16 * 8 * NRT + kThreads - 1) / kThreads;
+    constexpr int IPT = (CH * 8 * NRT + kThreads - 1) / kThreads;
     constexpr bool kAhead = IPT == 1;
     Stage stg[IPT];
     if constexpr (kAhead) {
-        if (tid < n_items) load_item<FORM>(tid, spectra, ys, nf, frames, 0, c0, stg[0]);
+        if (tid < n_items) St::load(tid, spectra, nf, fcap, spec_tile, frames, 0, c0, stg[0]);
     }
 
     for (int f0 = 0; f0 < frames; f0 += T::kFramesPerStep) {
         sync();
         if constexpr (kAhead) {
             if (tid < n_items) {
-                store_item<FORM>(tid, lds, cpitch, stg[0]);
+                St::store(tid, lds, cpitch, stg[0]);
                 if (f0 + T::kFramesPerStep < frames)
-                    load_item<FORM>(tid, spectra, ys, nf, frames, f0 + T::kFramesPerStep, c0, stg[0]);
+                    St::load(tid, spectra, nf, fcap, spec_tile, frames, f0 + T::kFramesPerStep, c0, stg[0]);
             }
         } else {
 #pragma unroll 1
             for (int item = tid; item < n_items; item += kThreads) {
-                load_item<FORM>(item, spectra, ys, nf, frames, f0, c0, stg[0]);
-                store_item<FORM>(item, lds, cpitch, stg[0]);
+                St::load(item, spectra, nf, fcap, spec_tile, frames, f0, c0, stg[0]);
+                St::store(item, lds, cpitch, stg[0]);
             }
         }
         sync();

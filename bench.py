@@ -173,6 +173,7 @@ def main():
     def run_form(name):
         form = FORMS[name]
         eng = corr.Engine(N, nf, chunk, form)
+        xtile = eng.xtile() if form != corr.FORM_FP32 else None
         acc = hip.Buffer(B * nf * eng.integrator_bytes())
         cal_ms = None
         if form != corr.FORM_FP32:
@@ -241,6 +242,9 @@ def main():
                                 "frac_hbm": round(x_gbs / HBM_PEAK_GBS, 4)},
             f"dump_{name}": {"avg_ms": round(ms["d"], 4)},
         }
+        if xtile is not None:
+            kern[f"xengine_{name}"]["channels_per_workgroup"] = xtile[0]
+            kern[f"xengine_{name}"]["spectra_tile"] = xtile[1]
         return {"ms_per_step": wall / args.steps * 1e3,
                 "value": 2 * nf * T / (wall / args.steps) / 1e6,
                 "ms": ms, "kernels": kern, "vis": vis, "calibrate_ms": cal_ms,

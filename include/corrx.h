@@ -86,13 +86,24 @@ corr_status corrx_calibrate(void *engine, const int8_t *d_in, size_t station_pit
 corr_status corrx_set_gains(void *engine, const float *d_gains, void *stream);
 corr_status corrx_gains(void *engine, const float **d_gains);
 
+/* FP16/INT8 forms: channels per X-engine workgroup (16 or 32; 32 only up to 24
+   inputs) and the channel tile of the spectrum buffer's order (that width, or
+   n_freq for the plain order). The engine picks both at creation by timing
+   F- and X-engine with every combination that fits its shape (cached per
+   process, form, input count, n_freq and device); set overrides the choice
+   (before the next corrx_fengine call), get reports it. */
+corr_status corrx_set_xtile(void *engine, int32_t channels, int32_t spectra_tile);
+corr_status corrx_get_xtile(void *engine, int32_t *channels, int32_t *spectra_tile);
+
 /* d_acc *= 1.0f/(float)frames over n_complex values. */
 corr_status corrx_scale(void *d_acc, size_t n_complex, uint64_t frames, void *stream);
 
-/* The engine's spectrum buffer: [station][chunk_frames][n_freq] elements of the
-   form: complex64 (FP32); two IEEE halves in 32 bits, real in the low half
-   (FP16); two int8 in 16 bits, real in the low byte (INT8). Each is the
-   spectrum times the gain of its (input, channel). */
+/* The engine's spectrum buffer. FP32: complex64 [station][chunk_frames][n_freq].
+   FP16 and INT8: [station][n_freq/T][chunk_frames][T] with T the spectra tile
+   of corrx_get_xtile (T = n_freq is the plain [station][frame][channel] order),
+   of two IEEE halves in 32 bits, real in the low half (FP16), or two int8 in 16
+   bits, real in the low byte (INT8), each the spectrum times the gain of its
+   (input, channel). station_stride_elems = chunk_frames * n_freq. */
 corr_status corrx_spectra(void *engine, void **d_spectra, size_t *station_stride_elems);
 
 #ifdef __cplusplus

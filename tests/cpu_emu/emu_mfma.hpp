@@ -15,9 +15,9 @@
 #include "xmfma_body.hpp"
 
 struct EmuMfmaShared {
-    corr::xmfma::Frag a[corr::xmfma::kThreads], b[corr::xmfma::kThreads];
-    float xf[corr::xmfma::kThreads];
-    int32_t xi[corr::xmfma::kThreads];
+    corr::xmfma::Frag a[corr::xmfma::kMaxThreads], b[corr::xmfma::kMaxThreads];
+    float xf[corr::xmfma::kMaxThreads];
+    int32_t xi[corr::xmfma::kMaxThreads];
 };
 
 struct EmuMfmaOps {

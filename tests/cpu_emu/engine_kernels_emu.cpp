@@ -54,39 +54,48 @@ hipError_t fengine_t(const int8_t *in, size_t station_pitch, const int32_t *vali
 template <int LOG2N, int FORM>
 hipError_t fengine_form_t(const int8_t *in, size_t station_pitch, const int32_t *valid, size_t valid_stride,
                           const float2 *tw, const float2 *wl, const float *gain, void *out,
-                          size_t out_station_stride, int frames, int n_station) {
+                          size_t out_station_stride, int tile_shift, int frames, int n_station) {
     constexpr int N = 1 << LOG2N;
     run_grid(frames, n_station, N / 16, fft::fengine_lds_elems<LOG2N>() * sizeof(float2),
              [=](int j, int bx, int by, char *lds, HostSync &sync) {
                  fft::fengine_form_block<LOG2N, FORM>(j, bx, by, reinterpret_cast<float2 *>(lds), sync, in,
                                                       station_pitch, valid, valid_stride, tw, wl, gain, out,
-                                                      out_station_stride);
+                                                      out_station_stride, tile_shift);
              });
     return hipSuccess;
 }
 
-template <int FORM, int NRT>
-void xmfma_t(int n_station, const void *spectra, size_t ys, int nf, int frames, void *integ) {
-    const int n_tiles = nf / xmfma::kChannels;
+template <int FORM, int NRT, int CH>
+void xmfma_t(int n_station, const void *spectra, size_t ys, int nf, int spec_tile, int frames, void *integ) {
+    const int n_tiles = nf / CH;
     EmuMfmaShared *sh = new EmuMfmaShared;
-    run_grid(n_tiles, 1, xmfma::kThreads, xmfma::lds_bytes(n_station),
+    run_grid(n_tiles, 1, xmfma::Tile<CH>::kThreads, xmfma::lds_bytes(n_station, CH),
              [=](int t, int bx, int, char *lds, HostSync &sync) {
                  EmuMfmaOps ops{t, sh, &sync};
-                 xmfma::xengine_mfma_block<FORM, NRT>(t, bx, n_tiles, reinterpret_cast<uint8_t *>(lds), sync,
-                                                      ops, spectra, ys, nf, frames, n_station, integ);
+                 xmfma::xengine_mfma_block<FORM, NRT, CH>(t, bx, n_tiles, reinterpret_cast<uint8_t *>(lds), sync,
+                                                          ops, spectra, ys, nf, spec_tile, frames, n_station, integ);
              });
     delete sh;
 }
 
 template <int FORM>
-hipError_t xmfma_f(int n_station, const void *spectra, size_t ys, int nf, int frames, void *integ) {
-    switch (xmfma::row_tiles(n_station)) {
-    case 1: xmfma_t<FORM, 1>(n_station, spectra, ys, nf, frames, integ); break;
-    case 2: xmfma_t<FORM, 2>(n_station, spectra, ys, nf, frames, integ); break;
-    case 3: xmfma_t<FORM, 3>(n_station, spectra, ys, nf, frames, integ); break;
-    case 4: xmfma_t<FORM, 4>(n_station, spectra, ys, nf, frames, integ); break;
-    case 5: xmfma_t<FORM, 5>(n_station, spectra, ys, nf, frames, integ); break;
-    case 6: xmfma_t<FORM, 6>(n_station, spectra, ys, nf, frames, integ); break;
+hipError_t xmfma_f(int n_station, int ch, const void *spectra, size_t ys, int nf, int tl, int frames, void *integ) {
+    const int nrt = xmfma::row_tiles(n_station);
+    if (ch == 32) {
+        switch (nrt) {
+        case 1: xmfma_t<FORM, 1, 32>(n_station, spectra, ys, nf, tl, frames, integ); return hipSuccess;
+        case 2: xmfma_t<FORM, 2, 32>(n_station, spectra, ys, nf, tl, frames, integ); return hipSuccess;
+        case 3: xmfma_t<FORM, 3, 32>(n_station, spectra, ys, nf, tl, frames, integ); return hipSuccess;
+        default: return hipErrorInvalidValue;
+        }
+    }
+    switch (nrt) {
+    case 1: xmfma_t<FORM, 1, 16>(n_station, spectra, ys, nf, tl, frames, integ); break;
+    case 2: xmfma_t<FORM, 2, 16>(n_station, spectra, ys, nf, tl, frames, integ); break;
+    case 3: xmfma_t<FORM, 3, 16>(n_station, spectra, ys, nf, tl, frames, integ); break;
+    case 4: xmfma_t<FORM, 4, 16>(n_station, spectra, ys, nf, tl, frames, integ); break;
+    case 5: xmfma_t<FORM, 5, 16>(n_station, spectra, ys, nf, tl, frames, integ); break;
+    case 6: xmfma_t<FORM, 6, 16>(n_station, spectra, ys, nf, tl, frames, integ); break;
     default: return hipErrorInvalidValue;
     }
     return hipSuccess;
@@ -153,16 +162,18 @@ hipError_t launch_xengine_fp32(const float2 *y, size_t y_station_stride, int n_f
 hipError_t launch_fengine_form(int form, int log2_nf, const int8_t *in, size_t station_pitch,
                                const int32_t *valid, size_t valid_stride, const float2 *tw,
                                const float2 *wl, const float *gain, void *out,
-                               size_t out_station_stride, int frames, int n_station, hipStream_t) {
+                               size_t out_station_stride, int tile, int frames, int n_station, hipStream_t) {
     if (frames <= 0 || n_station <= 0) return hipSuccess;
     if (form != 1 && form != 2) return hipErrorInvalidValue;
+    if (tile != 16 && tile != 32 && tile != (1 << log2_nf)) return hipErrorInvalidValue;
+    const int sh = tile == 16 ? 4 : tile == 32 ? 5 : log2_nf;
     switch (log2_nf) {
 #define E(L)                                                                                            \
     case L:                                                                                             \
         return form == 1 ? fengine_form_t<L, 1>(in, station_pitch, valid, valid_stride, tw, wl, gain, out, \
-                                                out_station_stride, frames, n_station)                  \
+                                                out_station_stride, sh, frames, n_station)              \
                          : fengine_form_t<L, 2>(in, station_pitch, valid, valid_stride, tw, wl, gain, out, \
-                                                out_station_stride, frames, n_station);
+                                                out_station_stride, sh, frames, n_station);
         E(9) E(10) E(11) E(12) E(13) E(14)
 #undef E
     default:
@@ -194,17 +205,17 @@ hipError_t launch_fill(void *p, size_t n_words, uint32_t v, hipStream_t) {
     return hipSuccess;
 }
 
-hipError_t xengine_mfma_prepare(int form, int n_station) {
-    return (form == 1 || form == 2) && xmfma::row_tiles(n_station) <= xmfma::kMaxRowTiles ? hipSuccess
-                                                                                           : hipErrorInvalidValue;
+hipError_t xengine_mfma_prepare(int form, int n_station, int ch) {
+    return (form == 1 || form == 2) && xmfma::tile_fits(n_station, ch) ? hipSuccess : hipErrorInvalidValue;
 }
 
-hipError_t launch_xengine_mfma(int form, int n_station, const void *spectra, size_t ys, int nf, int frames,
-                               void *integ, hipStream_t) {
+hipError_t launch_xengine_mfma(int form, int n_station, int ch, const void *spectra, size_t ys, int nf,
+                               int spec_tile, int frames, void *integ, hipStream_t) {
     if (frames <= 0) return hipSuccess;
-    if (nf % xmfma::kChannels != 0) return hipErrorInvalidValue;
-    if (form == 1) return xmfma_f<1>(n_station, spectra, ys, nf, frames, integ);
-    if (form == 2) return xmfma_f<2>(n_station, spectra, ys, nf, frames, integ);
+    if (!xmfma::tile_fits(n_station, ch) || nf % ch != 0) return hipErrorInvalidValue;
+    if (spec_tile != ch && spec_tile != nf) return hipErrorInvalidValue;
+    if (form == 1) return xmfma_f<1>(n_station, ch, spectra, ys, nf, spec_tile, frames, integ);
+    if (form == 2) return xmfma_f<2>(n_station, ch, spectra, ys, nf, spec_tile, frames, integ);
     return hipErrorInvalidValue;
 }
 

@@ -42,7 +42,7 @@ static void run_block(int f, int st, const int8_t *in, size_t pitch, const int32
 template <int LOG2N, int FORM>
 static void run_form_block(int f, int st, const int8_t *in, size_t pitch, const int32_t *valid,
                            size_t vstride, const float2 *tw, const float2 *wl, const float *gain,
-                           void *out, size_t ostride) {
+                           void *out, size_t ostride, int tile_shift) {
     constexpr int N = 1 << LOG2N;
     constexpr int NT = N / 16;
     float2 *lds = new float2[fengine_lds_elems<LOG2N>()];
@@ -53,7 +53,7 @@ static void run_form_block(int f, int st, const int8_t *in, size_t pitch, const 
         th.emplace_back([=, &bar] {
             HostSync sync{&bar};
             fengine_form_block<LOG2N, FORM>(j, f, st, lds, sync, in, pitch, valid, vstride, tw, wl, gain,
-                                            out, ostride);
+                                            out, ostride, tile_shift);
         });
     for (auto &t : th) t.join();
     delete[] lds;
@@ -77,21 +77,24 @@ static int check_forms(const int8_t *in, size_t pitch, const int32_t *valid, siz
     const size_t ostride = (size_t)fr * N;
     uint32_t *o16 = new uint32_t[ostride * nst];
     uint16_t *o8 = new uint16_t[ostride * nst];
-    for (int st = 0; st < nst; ++st)
-        for (int f = 0; f < fr; ++f) {
-            run_form_block<LOG2N, 1>(f, st, in, pitch, valid, vstride, tw, wl, g16, o16, ostride);
-            run_form_block<LOG2N, 2>(f, st, in, pitch, valid, vstride, tw, wl, g8, o8, ostride);
-        }
     int bad = 0;
-    for (int st = 0; st < nst; ++st)
-        for (int f = 0; f < fr; ++f)
-            for (int k = 0; k < N; ++k) {
-                const float2 y = ref[st * rstride + (size_t)f * N + k];
-                const size_t at = st * ostride + (size_t)f * N + k;
-                if (o16[at] != corr::forms::pack_fp16(y, g16[(size_t)st * N + k])) ++bad;
-                if (o8[at] != corr::forms::pack_int8(y, g8[(size_t)st * N + k])) ++bad;
+    for (int tile : {16, 32, N}) {  // N: the plain [station][frame][channel] order
+        const int sh = tile == 16 ? 4 : tile == 32 ? 5 : LOG2N;
+        for (int st = 0; st < nst; ++st)
+            for (int f = 0; f < fr; ++f) {
+                run_form_block<LOG2N, 1>(f, st, in, pitch, valid, vstride, tw, wl, g16, o16, ostride, sh);
+                run_form_block<LOG2N, 2>(f, st, in, pitch, valid, vstride, tw, wl, g8, o8, ostride, sh);
             }
-    std::printf("N=%d forms: %d mismatches\n", N, bad);
+        for (int st = 0; st < nst; ++st)
+            for (int f = 0; f < fr; ++f)
+                for (int k = 0; k < N; ++k) {
+                    const float2 y = ref[st * rstride + (size_t)f * N + k];
+                    const size_t at = corr::forms::spec_index(st, f, k, N, fr, tile);  // tile-major order
+                    if (o16[at] != corr::forms::pack_fp16(y, g16[(size_t)st * N + k])) ++bad;
+                    if (o8[at] != corr::forms::pack_int8(y, g8[(size_t)st * N + k])) ++bad;
+                }
+    }
+    std::printf("N=%d forms (tiles of 16, 32 and N): %d mismatches\n", N, bad);
     delete[] g16;
     delete[] g8;
     delete[] o16;

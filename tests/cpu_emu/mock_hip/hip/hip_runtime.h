@@ -55,6 +55,9 @@ hipError_t hipEventCreateWithFlags(hipEvent_t *e, unsigned f);
 hipError_t hipEventDestroy(hipEvent_t e);
 hipError_t hipEventRecord(hipEvent_t e, hipStream_t s);
 hipError_t hipEventSynchronize(hipEvent_t e);
+hipError_t hipEventCreate(hipEvent_t *e);
+hipError_t hipEventElapsedTime(float *ms, hipEvent_t a, hipEvent_t b);
+hipError_t hipGetDevice(int *d);
 hipError_t hipEventQuery(hipEvent_t e);
 hipError_t hipLaunchHostFunc(hipStream_t s, hipHostFn_t fn, void *user);
 hipError_t hipGetLastError(void);

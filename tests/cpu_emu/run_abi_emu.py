@@ -71,13 +71,12 @@ def forms(L, form, N, nf, T, chunk, cal_frames):
         e.fengine(ptr(x) + f0 * 2 * nf, pitch, n, ptr(mask), mask_pitch, f0)
         raw = np.ctypeslib.as_array(ctypes.cast(sp, ctypes.POINTER(ctypes.c_uint8)),
                                     shape=(N * stride * eb,)).reshape(N, stride * eb)
-        for s_ in range(N):
-            blk = raw[s_, : n * nf * eb]
-            if form == corr.FORM_FP16:
-                h = blk.view(np.float16).astype(np.float32).reshape(n, nf, 2)
-            else:
-                h = blk.view(np.int8).astype(np.float32).reshape(n, nf, 2)
-            Yq[s_, f0:f0 + n] = h[..., 0] + 1j * h[..., 1]
+        # tile-major order: [station][nf/tile][chunk frames][tile channels]
+        dt = np.float16 if form == corr.FORM_FP16 else np.int8
+        tl = e.xtile()[1]
+        h = raw.view(dt).astype(np.float32).reshape(N, nf // tl, stride // nf, tl, 2)[:, :, :n]
+        h = h.transpose(0, 2, 1, 3, 4).reshape(N, n, nf, 2)
+        Yq[:, f0:f0 + n] = h[..., 0] + 1j * h[..., 1]
         e.xengine(ptr(integ), n)
     out = np.zeros(B * nf, dtype=np.complex64)
     e.dump(ptr(integ), T, ptr(out))

@@ -1,5 +1,5 @@
-// CPU emulation of the MFMA X-engine kernel body (csrc/xmfma_body.hpp) with the
-// kernel's launch geometry: one host thread per lane, std::barrier for the
+// CPU emulation of the MFMA X-engine kernel body (csrc/xmfma_body.hpp), on
+// spectra in the tile-major order of forms::spec_index, with the kernel's launch geometry: one host thread per lane, std::barrier for the
 // workgroup barrier, the MFMA and lane exchange emulated with the kernel's lane
 // layout (emu_mfma.hpp), AddressSanitizer on every buffer. Checks against a
 // direct accumulation of the same spectra: int8 exactly (int64), FP16 against
@@ -15,20 +15,21 @@
 
 using namespace corr::xmfma;
 
-template <int FORM, int NRT>
-static void run(const void *y, size_t ys, int nf, int frames, int nst, void *integ) {
-    const int n_tiles = nf / kChannels;
-    const size_t shm = lds_bytes(nst);
+template <int FORM, int NRT, int CH>
+static void run(const void *y, size_t ys, int nf, int tl, int frames, int nst, void *integ) {
+    constexpr int T = Tile<CH>::kThreads;
+    const int n_tiles = nf / CH;
+    const size_t shm = lds_bytes(nst, CH);
     uint8_t *lds = new uint8_t[shm];  // heap, exact size: ASan sees the bounds
     EmuMfmaShared *sh = new EmuMfmaShared;
-    std::barrier<> bar(kThreads);
+    std::barrier<> bar(T);
     std::vector<std::thread> th;
-    for (int t = 0; t < kThreads; ++t)
+    for (int t = 0; t < T; ++t)
         th.emplace_back([=, &bar] {
             HostSync sync{&bar};
             EmuMfmaOps ops{t, sh, &sync};
             for (int bx = 0; bx < n_tiles; ++bx) {
-                xengine_mfma_block<FORM, NRT>(t, bx, n_tiles, lds, sync, ops, y, ys, nf, frames, nst, integ);
+                xengine_mfma_block<FORM, NRT, CH>(t, bx, n_tiles, lds, sync, ops, y, ys, nf, tl, frames, nst, integ);
                 sync();
             }
         });
@@ -38,18 +39,27 @@ static void run(const void *y, size_t ys, int nf, int frames, int nst, void *int
 }
 
 template <int FORM>
-static void dispatch(int nrt, const void *y, size_t ys, int nf, int frames, int nst, void *integ) {
+static void dispatch(int nrt, int ch, int tl, const void *y, size_t ys, int nf, int frames, int nst, void *integ) {
+    if (ch == 32) {
+        switch (nrt) {
+        case 1: run<FORM, 1, 32>(y, ys, nf, tl, frames, nst, integ); break;
+        case 2: run<FORM, 2, 32>(y, ys, nf, tl, frames, nst, integ); break;
+        case 3: run<FORM, 3, 32>(y, ys, nf, tl, frames, nst, integ); break;
+        }
+        return;
+    }
     switch (nrt) {
-    case 1: run<FORM, 1>(y, ys, nf, frames, nst, integ); break;
-    case 2: run<FORM, 2>(y, ys, nf, frames, nst, integ); break;
-    case 3: run<FORM, 3>(y, ys, nf, frames, nst, integ); break;
-    case 4: run<FORM, 4>(y, ys, nf, frames, nst, integ); break;
-    case 5: run<FORM, 5>(y, ys, nf, frames, nst, integ); break;
-    case 6: run<FORM, 6>(y, ys, nf, frames, nst, integ); break;
+    case 1: run<FORM, 1, 16>(y, ys, nf, tl, frames, nst, integ); break;
+    case 2: run<FORM, 2, 16>(y, ys, nf, tl, frames, nst, integ); break;
+    case 3: run<FORM, 3, 16>(y, ys, nf, tl, frames, nst, integ); break;
+    case 4: run<FORM, 4, 16>(y, ys, nf, tl, frames, nst, integ); break;
+    case 5: run<FORM, 5, 16>(y, ys, nf, tl, frames, nst, integ); break;
+    case 6: run<FORM, 6, 16>(y, ys, nf, tl, frames, nst, integ); break;
     }
 }
 
-static int check_int8(int nst, int nf, int frames, int cap) {
+static int check_int8(int nst, int nf, int frames, int cap, int ch = 16, bool plain = false) {
+    const int tl = plain ? nf : ch;
     const size_t ys = (size_t)cap * nf;
     std::mt19937 rng(nst * 7 + frames);
     std::uniform_int_distribution<int> d(-7, 7);
@@ -59,14 +69,15 @@ static int check_int8(int nst, int nf, int frames, int cap) {
     std::vector<int2> acc((size_t)B * nf);
     for (size_t i = 0; i < acc.size(); ++i) acc[i] = make_int2((int)(i % 97) - 40, 3 - (int)(i % 13));
     const std::vector<int2> start = acc;
-    dispatch<2>(row_tiles(nst), y.data(), ys, nf, frames, nst, acc.data());
+    dispatch<2>(row_tiles(nst), ch, tl, y.data(), ys, nf, frames, nst, acc.data());
     int bad = 0, b = 0;
     for (int j = 0; j < nst; ++j)
         for (int k = j; k < nst; ++k, ++b)
             for (int c = 0; c < nf; ++c) {
                 int64_t sr = start[(size_t)b * nf + c].x, si = start[(size_t)b * nf + c].y;
                 for (int t = 0; t < frames; ++t) {
-                    const uint16_t p = y[j * ys + (size_t)t * nf + c], q = y[k * ys + (size_t)t * nf + c];
+                    const uint16_t p = y[corr::forms::spec_index(j, t, c, nf, cap, tl)];
+                    const uint16_t q = y[corr::forms::spec_index(k, t, c, nf, cap, tl)];
                     const int xr = (int8_t)(p & 0xFF), xi = (int8_t)(p >> 8);
                     const int yr = (int8_t)(q & 0xFF), yi = (int8_t)(q >> 8);
                     sr += xr * yr + xi * yi;
@@ -75,11 +86,12 @@ static int check_int8(int nst, int nf, int frames, int cap) {
                 const int2 g = acc[(size_t)b * nf + c];
                 if (g.x != sr || g.y != si) ++bad;
             }
-    std::printf("int8 N=%d nf=%d frames=%d: %d mismatches\n", nst, nf, frames, bad);
+    std::printf("int8 N=%d nf=%d frames=%d ch=%d tile=%d: %d mismatches\n", nst, nf, frames, ch, tl, bad);
     return bad;
 }
 
-static int check_fp16(int nst, int nf, int frames, int cap) {
+static int check_fp16(int nst, int nf, int frames, int cap, int ch = 16, bool plain = false) {
+    const int tl = plain ? nf : ch;
     const size_t ys = (size_t)cap * nf;
     std::mt19937 rng(nst * 11 + frames);
     std::normal_distribution<float> d(0.0f, 200.0f);
@@ -88,7 +100,7 @@ static int check_fp16(int nst, int nf, int frames, int cap) {
     const int B = nst * (nst + 1) / 2;
     std::vector<float4> acc((size_t)B * nf);
     for (size_t i = 0; i < acc.size(); ++i) acc[i] = float4{1000.0f, -500.0f, 0.25f, -0.125f};
-    dispatch<1>(row_tiles(nst), y.data(), ys, nf, frames, nst, acc.data());
+    dispatch<1>(row_tiles(nst), ch, tl, y.data(), ys, nf, frames, nst, acc.data());
     double worst = 0;
     int bad = 0, b = 0;
     for (int j = 0; j < nst; ++j)
@@ -96,7 +108,8 @@ static int check_fp16(int nst, int nf, int frames, int cap) {
             for (int c = 0; c < nf; ++c) {
                 double sr = 1000.25, si = -500.125, mag = 1.0;
                 for (int t = 0; t < frames; ++t) {
-                    const uint32_t p = y[j * ys + (size_t)t * nf + c], q = y[k * ys + (size_t)t * nf + c];
+                    const uint32_t p = y[corr::forms::spec_index(j, t, c, nf, cap, tl)];
+                    const uint32_t q = y[corr::forms::spec_index(k, t, c, nf, cap, tl)];
                     const double xr = corr::forms::f16_bits_to_f32((uint16_t)p);
                     const double xi = corr::forms::f16_bits_to_f32((uint16_t)(p >> 16));
                     const double yr = corr::forms::f16_bits_to_f32((uint16_t)q);
@@ -110,7 +123,8 @@ static int check_fp16(int nst, int nf, int frames, int cap) {
                 if (e > worst) worst = e;
                 if (!(e < 2e-6)) ++bad;
             }
-    std::printf("fp16 N=%d nf=%d frames=%d: worst %.3e, %d bad\n", nst, nf, frames, worst, bad);
+    std::printf("fp16 N=%d nf=%d frames=%d ch=%d tile=%d: worst %.3e, %d bad\n", nst, nf, frames, ch, tl, worst,
+                bad);
     return bad;
 }
 
@@ -128,6 +142,16 @@ int main() {
     bad += check_fp16(8, 48, 64, 64);
     bad += check_fp16(24, 256, 40, 40);
     bad += check_fp16(40, 48, 31, 31);
+    // 32 channels per workgroup (up to 24 inputs): nf 512 = 16 tiles, nf 96 = 3
+    bad += check_int8(3, 96, 70, 70, 32);
+    bad += check_int8(24, 512, 40, 40, 32);
+    bad += check_fp16(13, 96, 33, 33, 32);
+    bad += check_fp16(24, 512, 40, 40, 32);
+    // the plain [input][frame][channel] order (a spectrum tile of n_freq)
+    bad += check_int8(9, 96, 70, 72, 16, true);
+    bad += check_int8(24, 512, 40, 40, 32, true);
+    bad += check_fp16(5, 512, 33, 40, 16, true);
+    bad += check_fp16(24, 96, 40, 40, 32, true);
     std::printf(bad ? "FAILED\n" : "OK\n");
     return bad ? 1 : 0;
 }

@@ -3,8 +3,9 @@
 MFMA X-engine alone, on spectra written straight into the engine's spectrum
 buffer: int8 bit-exact against an int64 sum of the same products, FP16 within
 FP32 rounding of an FP64 sum; the integrator's start values are kept (+=), for
-1..48 inputs (every row-tile count the kernel has), ragged frame counts and both
-channel-tile mappings.
+1..48 inputs (every row-tile count the kernel has), ragged frame counts, both
+channel-tile mappings and both workgroup widths (16 and 32 channels; the tuner's
+choice and each width forced).
 
 Whole form pipeline (calibrate, F-engine per chunk with a missing frame,
 X-engine, FP64 dump) against the FP64 oracle: on the engine's own requantized
@@ -35,17 +36,30 @@ def _upload_spectra(hip, eng, host_words):
     hip.check(hip.rt().hipMemcpy(ptr, a.ctypes.data, a.nbytes, hip.H2D), "hipMemcpy")
 
 
+def _to_tiles(a, fcap, tile):
+    """(N, T, nf, ...) spectra -> the engine's tile-major order
+    (N, nf/tile, fcap, tile, ...), frames past T zero, flattened per station."""
+    N, T, nf = a.shape[:3]
+    out = np.zeros((N, nf // tile, fcap, tile) + a.shape[3:], dtype=a.dtype)
+    out[:, :, :T] = a.reshape((N, T, nf // tile, tile) + a.shape[3:]).swapaxes(1, 2)
+    return out.reshape(N, -1)
+
+
+def _from_tiles(a, nf, fcap, T, tile):
+    """Inverse of _to_tiles for the first T frames: (N, ...) -> (N, T, nf, ...)."""
+    N = a.shape[0]
+    t = a.reshape((N, nf // tile, fcap, tile) + a.shape[2:])[:, :, :T]
+    return t.swapaxes(1, 2).reshape((N, T, nf) + a.shape[2:])
+
+
 def _download_spectra(hip, eng, N, T, nf, form):
     ptr, stride = eng.spectra()
     eb = 4 if form == 1 else 2
     raw = np.empty((N, stride * eb), dtype=np.uint8)
     hip.synchronize()
     hip.check(hip.rt().hipMemcpy(raw.ctypes.data, ptr, raw.nbytes, hip.D2H), "hipMemcpy")
-    blk = raw[:, : T * nf * eb]
-    if form == 1:
-        h = blk.view(np.float16).astype(np.float32).reshape(N, T, nf, 2)
-    else:
-        h = blk.view(np.int8).astype(np.float32).reshape(N, T, nf, 2)
+    dt = np.float16 if form == 1 else np.int8
+    h = _from_tiles(raw.view(dt).astype(np.float32).reshape(N, stride, 2), nf, stride // nf, T, eng.xtile()[1])
     return (h[..., 0] + 1j * h[..., 1]).astype(np.complex64)
 
 
@@ -60,17 +74,49 @@ def _x_int64(q, N):
     return np.stack(out)
 
 
-@pytest.mark.parametrize("N,nf,T", [(1, 512, 3), (4, 512, 70), (9, 1024, 130), (24, 512, 64),
-                                    (24, 16384, 37), (30, 512, 20), (40, 512, 65), (48, 512, 7)])
-def test_xengine_int8_exact(corr, hip, N, nf, T):
-    print(f"int8 MFMA X-engine N={N} nf={nf} T={T}", flush=True)
+# (N, nf, T, channels per workgroup, spectrum tile): 0 = the tuner's choice,
+# tile -1 = n_freq (the plain order)
+INT8_CASES = [(1, 512, 3, 0, 0), (4, 512, 70, 0, 0), (9, 1024, 130, 0, 0), (24, 512, 64, 0, 0),
+              (24, 16384, 37, 0, 0), (30, 512, 20, 0, 0), (40, 512, 65, 0, 0), (48, 512, 7, 0, 0),
+              (1, 512, 3, 32, 32), (5, 1024, 70, 32, -1), (16, 512, 64, 32, 32), (24, 16384, 37, 32, 32),
+              (24, 16384, 37, 16, -1), (24, 512, 130, 16, 16), (40, 1024, 65, 16, -1)]
+FP16_CASES = [(1, 512, 1, 0, 0), (5, 512, 33, 0, 0), (8, 1024, 64, 0, 0), (24, 512, 96, 0, 0),
+              (24, 16384, 40, 0, 0), (40, 512, 31, 0, 0), (48, 512, 50, 0, 0),
+              (3, 512, 33, 32, 32), (24, 16384, 40, 32, 32), (24, 16384, 40, 32, -1), (24, 512, 96, 16, 16),
+              (48, 512, 50, 16, -1)]
+
+
+def _engine(corr, N, nf, T, form, ch, tile):
+    eng = corr.Engine(N, nf, T, form)
+    c, t = eng.xtile()
+    assert c in ((16, 32) if N <= 24 else (16,)) and t in (c, nf)
+    if ch:
+        eng.set_xtile(ch, nf if tile < 0 else tile)
+        assert eng.xtile() == (ch, nf if tile < 0 else tile)
+    return eng
+
+
+def test_xtile_rejections(corr, hip):
+    eng = corr.Engine(40, 512, 8, corr.FORM_FP16)
+    assert eng.xtile()[0] == 16
+    with pytest.raises(corr.CorrError):
+        eng.set_xtile(32, 32)  # 32 channels hold at most 24 inputs
+    with pytest.raises(corr.CorrError):
+        eng.set_xtile(24, 24)
+    with pytest.raises(corr.CorrError):
+        eng.set_xtile(16, 32)  # the spectrum tile is the width or n_freq
+    eng.close()
+
+
+@pytest.mark.parametrize("N,nf,T,ch,tile", INT8_CASES)
+def test_xengine_int8_exact(corr, hip, N, nf, T, ch, tile):
+    print(f"int8 MFMA X-engine N={N} nf={nf} T={T} ch={ch} tile={tile}", flush=True)
     rng = np.random.default_rng(N * 1000 + T)
-    eng = corr.Engine(N, nf, T, corr.FORM_INT8)
+    eng = _engine(corr, N, nf, T, corr.FORM_INT8, ch, tile)
     _, stride = eng.spectra()
     q = rng.integers(-7, 8, size=(N, T, nf, 2)).astype(np.int8)
     q[0, 0, : min(nf, 4)] = [[7, -7], [-7, 7], [7, 7], [-7, -7]][: min(nf, 4)]
-    words = np.zeros((N, stride), dtype=np.uint16)
-    words[:, : T * nf] = q.view(np.uint16).reshape(N, T * nf)
+    words = _to_tiles(q.view(np.uint16).reshape(N, T, nf), stride // nf, eng.xtile()[1])
     _upload_spectra(hip, eng, words)
     B = N * (N + 1) // 2
     start = rng.integers(-1000, 1000, size=(B, nf, 2)).astype(np.int32)
@@ -88,16 +134,14 @@ def test_xengine_int8_exact(corr, hip, N, nf, T):
     eng.close()
 
 
-@pytest.mark.parametrize("N,nf,T", [(1, 512, 1), (5, 512, 33), (8, 1024, 64), (24, 512, 96),
-                                    (24, 16384, 40), (40, 512, 31), (48, 512, 50)])
-def test_xengine_fp16(corr, hip, N, nf, T):
-    print(f"FP16 MFMA X-engine N={N} nf={nf} T={T}", flush=True)
+@pytest.mark.parametrize("N,nf,T,ch,tile", FP16_CASES)
+def test_xengine_fp16(corr, hip, N, nf, T, ch, tile):
+    print(f"FP16 MFMA X-engine N={N} nf={nf} T={T} ch={ch} tile={tile}", flush=True)
     rng = np.random.default_rng(N * 77 + T)
-    eng = corr.Engine(N, nf, T, corr.FORM_FP16)
+    eng = _engine(corr, N, nf, T, corr.FORM_FP16, ch, tile)
     _, stride = eng.spectra()
     h = rng.normal(0.0, 200.0, size=(N, T, nf, 2)).astype(np.float16)
-    words = np.zeros((N, stride), dtype=np.uint32)
-    words[:, : T * nf] = h.view(np.uint32).reshape(N, T * nf)
+    words = _to_tiles(h.view(np.uint32).reshape(N, T, nf), stride // nf, eng.xtile()[1])
     _upload_spectra(hip, eng, words)
     B = N * (N + 1) // 2
     start = np.zeros((B, nf, 4), dtype=np.float32)
