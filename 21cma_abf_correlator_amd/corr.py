@@ -74,6 +74,20 @@ class corrx_shape(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in ("n_station", "n_freq", "chunk_frames", "form")]
 
 
+BATCH_ZERO = 1
+BATCH_DUMP = 2
+
+
+class corrx_batch_item(ctypes.Structure):
+    _fields_ = [
+        ("d_in", ctypes.c_void_p), ("station_pitch", ctypes.c_size_t),
+        ("d_mask", ctypes.c_void_p), ("mask_pitch", ctypes.c_size_t),
+        ("mask_bit0", ctypes.c_int32), ("flags", ctypes.c_int32),
+        ("frames", ctypes.c_int64), ("d_acc", ctypes.c_void_p),
+        ("dump_frames", ctypes.c_uint64), ("d_out", ctypes.c_void_p),
+    ]
+
+
 NOTIFY_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p)
 
 # Every symbol include/corr_cuda.h and include/corrx.h declare.
@@ -88,7 +102,7 @@ ABI_SYMBOLS = (
 )
 ENGINE_SYMBOLS = (
     "corrx_engine_create", "corrx_engine_destroy", "corrx_fengine", "corrx_xengine",
-    "corrx_scale", "corrx_spectra", "corrx_integrator_bytes", "corrx_dump", "corrx_calibrate",
+    "corrx_integrate", "corrx_batch_create", "corrx_batch_destroy", "corrx_batch_run", "corrx_scale", "corrx_spectra", "corrx_integrator_bytes", "corrx_dump", "corrx_calibrate",
     "corrx_set_gains", "corrx_gains", "corrx_set_xtile", "corrx_get_xtile",
 )
 
@@ -165,6 +179,10 @@ def declare(L):
         "corrx_engine_destroy": ([vp], st),
         "corrx_fengine": ([vp, vp, sz, vp, sz, i32, i32, vp], st),
         "corrx_xengine": ([vp, vp, i32, vp], st),
+        "corrx_integrate": ([vp, vp, sz, vp, sz, i32, ctypes.c_int64, vp, vp], st),
+        "corrx_batch_create": ([ctypes.POINTER(vp), i32, ctypes.POINTER(vp)], st),
+        "corrx_batch_destroy": ([vp], st),
+        "corrx_batch_run": ([vp, ctypes.POINTER(corrx_batch_item), i32, vp], st),
         "corrx_scale": ([vp, sz, u64, vp], st),
         "corrx_spectra": ([vp, ctypes.POINTER(vp), ctypes.POINTER(sz)], st),
         "corrx_integrator_bytes": ([vp, ctypes.POINTER(sz)], st),
@@ -393,6 +411,13 @@ class Engine:
     def xengine(self, d_acc, frames, stream=None):
         check(self._lib.corrx_xengine(self.h, d_acc, frames, stream), "corrx_xengine")
 
+    def integrate(self, d_in, station_pitch, frames, d_acc, d_mask=None, mask_pitch=0, mask_bit0=0,
+                  stream=None):
+        """F- and X-engine over `frames` frames (any count) into d_acc, the
+        X-engine of each run of chunk_frames overlapping the next F-engine."""
+        check(self._lib.corrx_integrate(self.h, d_in, station_pitch, d_mask, mask_pitch, mask_bit0,
+                                        frames, d_acc, stream), "corrx_integrate")
+
     def spectra(self):
         p = ctypes.c_void_p()
         s = ctypes.c_size_t()
@@ -430,6 +455,42 @@ class Engine:
 
     def set_xtile(self, channels, spectra_tile):
         check(self._lib.corrx_set_xtile(self.h, channels, spectra_tile), "corrx_set_xtile")
+
+
+class Batch:
+    """One dump of several engines (independent sub-arrays) at once
+    (corrx_batch_*): per engine, zero + integrate + dump on a branch stream of
+    its own; optionally captured into a hipGraph and replayed."""
+
+    def __init__(self, engines, lib=None):
+        self._lib = lib if lib is not None else load()
+        self.engines = list(engines)
+        arr = (ctypes.c_void_p * len(self.engines))(*[e.h for e in self.engines])
+        h = ctypes.c_void_p()
+        check(self._lib.corrx_batch_create(arr, len(self.engines), ctypes.byref(h)), "corrx_batch_create")
+        self.h = h.value
+
+    def close(self):
+        if self.h is not None:
+            self._lib.corrx_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def item(d_in, station_pitch, frames, d_acc, d_out=None, dump_frames=0, d_mask=None,
+             mask_pitch=0, mask_bit0=0, zero=True):
+        flags = (BATCH_ZERO if zero else 0) | (BATCH_DUMP if d_out is not None else 0)
+        return corrx_batch_item(d_in, station_pitch, d_mask, mask_pitch, mask_bit0, flags, frames,
+                                d_acc, dump_frames, d_out)
+
+    def run(self, items, graph=True, stream=None):
+        arr = (corrx_batch_item * len(items))(*items)
+        check(self._lib.corrx_batch_run(self.h, arr, 1 if graph else 0, stream), "corrx_batch_run")
 
 
 def scale(d_acc, n_complex, frames, stream=None):

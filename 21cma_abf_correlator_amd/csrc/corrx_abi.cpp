@@ -1,9 +1,13 @@
 // corrx_abi.cpp -- C entry points of the device engine (include/corrx.h).
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
+#include "batch.hpp"
 #include "corrx.h"
 #include "engine.hpp"
 
+using corr::Batch;
 using corr::Engine;
 using corr::fail;
 
@@ -44,6 +48,14 @@ extern "C" corr_status corrx_fengine(void *engine, const int8_t *d_in, size_t st
 extern "C" corr_status corrx_xengine(void *engine, void *d_acc, int32_t frames, void *stream) {
     if (engine == nullptr) return fail(CORR_ERR_ARG, "engine is NULL");
     return as_engine(engine)->xengine(d_acc, frames, as_stream(stream));
+}
+
+extern "C" corr_status corrx_integrate(void *engine, const int8_t *d_in, size_t station_pitch,
+                                       const uint8_t *d_mask, size_t mask_pitch, int32_t mask_bit0,
+                                       int64_t frames, void *d_acc, void *stream) {
+    if (engine == nullptr) return fail(CORR_ERR_ARG, "engine is NULL");
+    return as_engine(engine)->integrate(d_in, station_pitch, d_mask, mask_pitch, mask_bit0, frames, d_acc,
+                                        as_stream(stream));
 }
 
 extern "C" corr_status corrx_integrator_bytes(void *engine, size_t *bytes) {
@@ -105,4 +117,28 @@ extern "C" corr_status corrx_get_xtile(void *engine, int32_t *channels, int32_t 
     *channels = as_engine(engine)->xtile();
     *spectra_tile = as_engine(engine)->spec_tile();
     return CORR_OK;
+}
+
+extern "C" corr_status corrx_batch_create(void *const *engines, int32_t n_engines, void **out_batch) {
+    if (out_batch == nullptr) return fail(CORR_ERR_ARG, "out_batch is NULL");
+    *out_batch = nullptr;
+    if (engines == nullptr || n_engines < 1) return fail(CORR_ERR_ARG, "batch: needs at least one engine");
+    std::vector<Engine *> e((size_t)n_engines);
+    for (int32_t i = 0; i < n_engines; ++i) e[(size_t)i] = as_engine(engines[i]);
+    Batch *b = nullptr;
+    const corr_status st = Batch::create(e.data(), n_engines, &b);
+    if (st != CORR_OK) return st;
+    *out_batch = b;
+    return CORR_OK;
+}
+
+extern "C" corr_status corrx_batch_destroy(void *batch) {
+    delete static_cast<Batch *>(batch);
+    return CORR_OK;
+}
+
+extern "C" corr_status corrx_batch_run(void *batch, const corrx_batch_item *items, int32_t use_graph,
+                                       void *stream) {
+    if (batch == nullptr) return fail(CORR_ERR_ARG, "batch is NULL");
+    return static_cast<Batch *>(batch)->run(items, use_graph != 0, as_stream(stream));
 }

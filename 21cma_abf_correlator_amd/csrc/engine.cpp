@@ -273,6 +273,15 @@ corr_status Engine::set_xtile(int channels, int spec_tile) {
 }
 
 Engine::~Engine() {
+    if (xs_ != nullptr) {
+        hipStreamSynchronize(xs_);
+        hipStreamDestroy(xs_);
+    }
+    for (int i = 0; i < 2; ++i) {
+        if (ev_f_[i] != nullptr) hipEventDestroy(ev_f_[i]);
+        if (ev_x_[i] != nullptr) hipEventDestroy(ev_x_[i]);
+    }
+    hipFree(d_spec_b_);
     hipFree(d_tw_);
     hipFree(d_wl_);
     hipFree(d_spec_);
@@ -302,6 +311,89 @@ size_t Engine::integrator_elem_bytes() const {
 
 corr_status Engine::fengine(const int8_t *d_in, size_t station_pitch, const uint8_t *d_mask,
                             size_t mask_pitch, int mask_bit0, int frames, hipStream_t s) {
+    return fengine_to(d_spec_, d_in, station_pitch, d_mask, mask_pitch, mask_bit0, frames, s);
+}
+
+corr_status Engine::xengine(void *integ, int frames, hipStream_t s) {
+    return xengine_from(d_spec_, integ, frames, s);
+}
+
+corr_status Engine::ensure_pipeline() {
+    if (xs_ != nullptr) return CORR_OK;
+    const size_t bytes = (size_t)shape_.n_station * spectra_station_stride() * spectra_elem_bytes();
+    corr_status st;
+    if (d_spec_b_ == nullptr &&
+        (st = hip_check(hipMalloc(&d_spec_b_, bytes), "hipMalloc(second spectrum buffer)")) != CORR_OK)
+        return st;
+    for (int i = 0; i < 2; ++i) {
+        if (ev_f_[i] == nullptr &&
+            (st = hip_check(hipEventCreateWithFlags(&ev_f_[i], hipEventDisableTiming), "hipEventCreate")) != CORR_OK)
+            return st;
+        if (ev_x_[i] == nullptr &&
+            (st = hip_check(hipEventCreateWithFlags(&ev_x_[i], hipEventDisableTiming), "hipEventCreate")) != CORR_OK)
+            return st;
+    }
+    return hip_check(hipStreamCreateWithFlags(&xs_, hipStreamNonBlocking), "hipStreamCreate(X-engine)");
+}
+
+// Chunk k: F-engine into buffer k%2 on `s` (after the X-engine of chunk k-2,
+// which read that buffer), then the X-engine on xs_ after it. The X-engines
+// run in chunk order on xs_, so the integrator sees one writer at a time.
+corr_status Engine::integrate(const int8_t *d_in, size_t station_pitch, const uint8_t *d_mask,
+                              size_t mask_pitch, int mask_bit0, int64_t frames, void *integ, hipStream_t s) {
+    if (frames < 0) return fail(CORR_ERR_ARG, "integrate: negative frame count");
+    if (d_in == nullptr) return fail(CORR_ERR_ARG, "integrate: input is NULL");
+    if (integ == nullptr) return fail(CORR_ERR_ARG, "integrate: integrator is NULL");
+    if (station_pitch / (2u * (size_t)shape_.n_freq) < (uint64_t)frames)
+        return fail(CORR_ERR_ARG, "integrate: station pitch shorter than the frames it holds");
+    if (mask_bit0 < 0) return fail(CORR_ERR_ARG, "integrate: negative mask bit offset");
+    if (d_mask != nullptr && mask_pitch * 8u < (uint64_t)mask_bit0 + (uint64_t)frames)
+        return fail(CORR_ERR_ARG, "integrate: mask pitch shorter than the bits it must hold");
+    if (d_mask != nullptr && (uint64_t)mask_bit0 + (uint64_t)frames > (uint64_t)INT32_MAX)
+        return fail(CORR_ERR_ARG, "integrate: mask bits beyond 2^31");
+    if (frames == 0) return CORR_OK;
+    const int chunk = shape_.chunk_frames;
+    corr_status st = CORR_OK;
+    if (!overlaps(frames)) {
+        // one stream: the FP32 form's X-engine is VALU-bound like the F-engine,
+        // and beside it both run slower than in turn; one run has no overlap
+        for (int64_t f0 = 0; f0 < frames && st == CORR_OK; f0 += chunk) {
+            const int n = frames - f0 < chunk ? (int)(frames - f0) : chunk;
+            const int bit0 = d_mask != nullptr ? mask_bit0 + (int)f0 : 0;
+            if ((st = fengine_to(d_spec_, d_in + (size_t)f0 * 2u * (size_t)shape_.n_freq, station_pitch, d_mask,
+                                 mask_pitch, bit0, n, s)) == CORR_OK)
+                st = xengine_from(d_spec_, integ, n, s);
+        }
+        return st;
+    }
+    if ((st = ensure_pipeline()) != CORR_OK) return st;
+    void *const spec[2] = {d_spec_, d_spec_b_};
+    int64_t k = 0;
+    int last = -1;  // buffer whose X-engine was enqueued last
+    for (int64_t f0 = 0; f0 < frames && st == CORR_OK; f0 += chunk, ++k) {
+        const int b = (int)(k & 1);
+        const int n = frames - f0 < chunk ? (int)(frames - f0) : chunk;
+        const int bit0 = d_mask != nullptr ? mask_bit0 + (int)f0 : 0;
+        if ((k < 2 || (st = hip_check(hipStreamWaitEvent(s, ev_x_[b], 0), "hipStreamWaitEvent")) == CORR_OK) &&
+            (st = fengine_to(spec[b], d_in + (size_t)f0 * 2u * (size_t)shape_.n_freq, station_pitch, d_mask,
+                             mask_pitch, bit0, n, s)) == CORR_OK &&
+            (st = hip_check(hipEventRecord(ev_f_[b], s), "hipEventRecord")) == CORR_OK &&
+            (st = hip_check(hipStreamWaitEvent(xs_, ev_f_[b], 0), "hipStreamWaitEvent")) == CORR_OK &&
+            (st = xengine_from(spec[b], integ, n, xs_)) == CORR_OK)
+            st = hip_check(hipEventRecord(ev_x_[b], xs_), "hipEventRecord");
+        if (st == CORR_OK) last = b;
+    }
+    // Join even after a failed launch, so that `s` never runs ahead of an
+    // X-engine already enqueued.
+    if (last >= 0) {
+        const corr_status j = hip_check(hipStreamWaitEvent(s, ev_x_[last], 0), "hipStreamWaitEvent");
+        if (st == CORR_OK) st = j;
+    }
+    return st;
+}
+
+corr_status Engine::fengine_to(void *spec, const int8_t *d_in, size_t station_pitch, const uint8_t *d_mask,
+                               size_t mask_pitch, int mask_bit0, int frames, hipStream_t s) {
     if (frames < 0 || frames > shape_.chunk_frames)
         return failf(CORR_ERR_ARG, "fengine: frames=%d outside [0, %d]", frames,
                      shape_.chunk_frames);
@@ -324,21 +416,26 @@ corr_status Engine::fengine(const int8_t *d_in, size_t station_pitch, const uint
     }
     if (shape_.form == Form::kFP32)
         return hip_check(launch_fengine_fp32(log2_nf_, d_in, station_pitch, valid, valid_stride, d_tw_,
-                                             d_wl_, static_cast<float2 *>(d_spec_),
+                                             d_wl_, static_cast<float2 *>(spec),
                                              spectra_station_stride(), frames, shape_.n_station, s),
                          "fengine_fp32_kernel");
     return hip_check(launch_fengine_form((int)shape_.form, log2_nf_, d_in, station_pitch, valid,
-                                         valid_stride, d_tw_, d_wl_, d_gain_, d_spec_,
+                                         valid_stride, d_tw_, d_wl_, d_gain_, spec,
                                          spectra_station_stride(), spec_tile_, frames, shape_.n_station, s),
                      "fengine_form_kernel");
 }
 
-corr_status Engine::xengine(void *integ, int frames, hipStream_t s) {
-    if (shape_.form == Form::kFP32) return xengine_fp32(static_cast<float2 *>(integ), frames, s);
+corr_status Engine::xengine_from(const void *spec, void *integ, int frames, hipStream_t s) {
     if (integ == nullptr) return fail(CORR_ERR_ARG, "xengine: integrator is NULL");
     if (frames < 0 || frames > shape_.chunk_frames)
         return failf(CORR_ERR_ARG, "xengine: frames=%d outside [0, %d]", frames, shape_.chunk_frames);
-    return hip_check(launch_xengine_mfma((int)shape_.form, shape_.n_station, x_tile_, d_spec_,
+    if (shape_.form == Form::kFP32)
+        return hip_check(launch_xengine_fp32(static_cast<const float2 *>(spec), spectra_station_stride(),
+                                             shape_.n_freq, frames, shape_.n_station, d_bpairs_, n_bpairs_,
+                                             x_npair_, x_waves_, x_groups_, x_frames_lds_,
+                                             static_cast<float2 *>(integ), s),
+                         "xengine_fp32_kernel");
+    return hip_check(launch_xengine_mfma((int)shape_.form, shape_.n_station, x_tile_, spec,
                                          spectra_station_stride(), shape_.n_freq, spec_tile_, frames, integ, s),
                      "xengine_mfma_kernel");
 }
@@ -399,15 +496,9 @@ corr_status Engine::set_gains(const float *d_gains, hipStream_t s) {
 }
 
 corr_status Engine::xengine_fp32(float2 *acc, int frames, hipStream_t s) {
+    if (shape_.form != Form::kFP32) return fail(CORR_ERR_ARG, "xengine_fp32: the engine is not of the FP32 form");
     if (acc == nullptr) return fail(CORR_ERR_ARG, "xengine: accumulator is NULL");
-    if (frames < 0 || frames > shape_.chunk_frames)
-        return failf(CORR_ERR_ARG, "xengine: frames=%d outside [0, %d]", frames,
-                     shape_.chunk_frames);
-    return hip_check(launch_xengine_fp32(static_cast<const float2 *>(d_spec_),
-                                         spectra_station_stride(), shape_.n_freq, frames,
-                                         shape_.n_station, d_bpairs_, n_bpairs_, x_npair_,
-                                         x_waves_, x_groups_, x_frames_lds_, acc, s),
-                     "xengine_fp32_kernel");
+    return xengine_from(d_spec_, acc, frames, s);
 }
 
 corr_status Engine::scale(float2 *acc, size_t n, uint64_t frames, hipStream_t s) {

@@ -81,6 +81,25 @@ class Engine {
     /// compensation re, compensation im) (FP16), int2 (int8).
     corr_status xengine(void *integ, int frames, hipStream_t s);
 
+    /// F- and X-engine over `frames` frames (any count, in runs of at most
+    /// chunk_frames), accumulated into `integ` as xengine does; same input
+    /// arguments as fengine. FP16/int8 forms with more than one run: the
+    /// F-engine runs on `s`, the X-engine on the engine's own stream, and the
+    /// spectra alternate between two buffers, so the X-engine of one run
+    /// overlaps the F-engine of the next. Otherwise (the FP32 form, or one run)
+    /// both run in turn on `s`. Every launch is ordered after the work already
+    /// enqueued on `s`, and `s` waits for the last X-engine before this
+    /// returns: work enqueued on `s` afterwards sees the whole integration. The
+    /// first overlapped call allocates the second spectrum buffer, the stream
+    /// and its events.
+    corr_status integrate(const int8_t *d_in, size_t station_pitch, const uint8_t *d_mask,
+                          size_t mask_pitch, int mask_bit0, int64_t frames, void *integ, hipStream_t s);
+
+    /// Allocate what integrate needs (done by its first overlapped call
+    /// otherwise): for callers that capture integrate into a graph, where
+    /// allocation is barred.
+    corr_status prepare_integrate() { return shape_.form == Form::kFP32 ? CORR_OK : ensure_pipeline(); }
+
     /// Bytes of one integrator element of this form.
     size_t integrator_elem_bytes() const;
 
@@ -123,6 +142,18 @@ class Engine {
   private:
     Engine() = default;
     corr_status init();
+
+    corr_status fengine_to(void *spec, const int8_t *d_in, size_t station_pitch, const uint8_t *d_mask,
+                           size_t mask_pitch, int mask_bit0, int frames, hipStream_t s);
+    corr_status xengine_from(const void *spec, void *integ, int frames, hipStream_t s);
+    corr_status ensure_pipeline();
+    bool overlaps(int64_t frames) const { return shape_.form != Form::kFP32 && frames > shape_.chunk_frames; }
+
+    // integrate(): second spectrum buffer, X-engine stream, per-buffer events.
+    void *d_spec_b_ = nullptr;
+    hipStream_t xs_ = nullptr;
+    hipEvent_t ev_f_[2] = {nullptr, nullptr};  ///< F-engine into buffer i done
+    hipEvent_t ev_x_[2] = {nullptr, nullptr};  ///< X-engine out of buffer i done
 
     EngineShape shape_{};
     int n_baseline_ = 0;

@@ -4,8 +4,11 @@
 Workload (one "step" = one integration, inputs already resident in HBM):
 24 int8 inputs, n_freq = 16384 (32768-sample real FFTs), 2048 spectra per
 integration: F-engine (mask + unpack + R2C FFT + the form's requantization) and
-X-engine over every chunk of frames, then the dump of the integration into
-complex64 visibilities. Synthetic correlated-Gaussian int8 data (sigma 16,
+X-engine over every chunk of frames (2048 frames per launch in the FP16 and int8
+forms, 512 in the FP32 form: the fastest measured for each), then the dump of
+the integration into complex64 visibilities. The step is one corrx_integrate
+call and the dump; the plain per-chunk loop is timed beside it and must give
+the same bytes. Synthetic correlated-Gaussian int8 data (sigma 16,
 rho 0.1, delays in [0, 32]): 64 distinct frames per input generated with numpy,
 tiled over the integration in HBM.
 
@@ -51,7 +54,9 @@ def parse():
     p.add_argument("--n-station", type=int, default=24)
     p.add_argument("--n-freq", type=int, default=16384)
     p.add_argument("--frames", type=int, default=2048, help="spectra per integration")
-    p.add_argument("--chunk", type=int, default=512, help="frames per engine launch")
+    p.add_argument("--chunk", type=int, default=0,
+                   help="frames per engine launch (0: 2048 for the FP16/int8 forms, 512 for FP32, "
+                        "at most --frames: the fastest measured for each)")
     p.add_argument("--rho", type=float, default=0.1)
     p.add_argument("--max-delay", type=int, default=32)
     p.add_argument("--forms", default="fp16,fp32,int8",
@@ -59,6 +64,9 @@ def parse():
     p.add_argument("--no-fp64", action="store_true", help="skip the FP64 error check")
     p.add_argument("--cpu-seconds", type=float, default=12.0,
                    help="target CPU time of the bounded CPU-baseline sample (0 = skip)")
+    p.add_argument("--no-batch", action="store_true",
+                   help="skip the batched three-sub-array workload (one hipGraph per dump)")
+    p.add_argument("--batch-steps", type=int, default=5)
     return p.parse_args()
 
 
@@ -121,6 +129,133 @@ def rel_rms(got, ref, rows=None):
     return float(np.sqrt(np.sum(np.abs(got - ref) ** 2) / np.sum(np.abs(ref) ** 2)))
 
 
+class ProfilerRegion:
+    """roctxProfilerResume(0) / roctxProfilerPause(0) around the per-kernel
+    timing pass, so that `rocprofv3 --marker-trace --selected-regions` records
+    exactly the launches whose HIP-event durations bench.py reports, and not
+    the engine's tile tuner, calibration or warm-up. Without a profiler attached
+    the calls do nothing; without the roctx library this is a no-op."""
+
+    def __init__(self):
+        import ctypes
+        self._lib = None
+        for name in ("librocprofiler-sdk-roctx.so.1", "/opt/rocm/lib/librocprofiler-sdk-roctx.so.1"):
+            try:
+                self._lib = ctypes.CDLL(name)
+                break
+            except OSError:
+                continue
+        if self._lib is not None:
+            for fn in (self._lib.roctxProfilerResume, self._lib.roctxProfilerPause):
+                fn.argtypes = [ctypes.c_uint64]
+                fn.restype = ctypes.c_int
+
+    def __enter__(self):
+        if self._lib is not None:
+            self._lib.roctxProfilerResume(0)
+        return self
+
+    def __exit__(self, *exc):
+        if self._lib is not None:
+            self._lib.roctxProfilerPause(0)
+        return False
+
+
+def resident_input(hip, pool_host, T):
+    """Device buffer [N][T][L] int8 holding the (N, P, L) pool tiled T/P times."""
+    N, P, L = pool_host.shape
+    pool = hip.Buffer.from_array(pool_host)
+    x = hip.Buffer(N * T * L)
+    for i in range(N):
+        for c in range(T // P):
+            hip.copy_d2d(x.ptr + (i * T + c * P) * L, pool.ptr + i * P * L, P * L)
+    pool.free()
+    return x
+
+
+# The batched workload: three independent sub-arrays of uneven shapes, one dump
+# of each per run. (inputs, n_freq, spectra per integration, rho, seed, frames
+# per launch)
+BATCH_SUBARRAYS = [(20, 8192, 512, 0.3, 501, 512), (24, 16384, 2048, 0.1, 502, 2048),
+                   (4, 16384, 8192, 0.5, 503, 2048)]
+
+
+def batch_c5(corr, hip, args, fp64):
+    """Three sub-arrays (FP16 form) in one corrx_batch_run per dump, replayed as
+    a hipGraph, against the same work run one sub-array after the other on one
+    stream. Reports time per batch run, total sample rate, the real-time factor
+    (800 Msamples/s per input) and each sub-array's relative RMS vs FP64."""
+    import numpy as np
+    stream = hip.Stream()
+    sp = stream.handle
+    subs = []
+    for N, nf, T, rho, seed, chunk in BATCH_SUBARRAYS:
+        L = 2 * nf
+        B = N * (N + 1) // 2
+        P = min(64, T)
+        pool = gen_pool(N, P, L, rho, 16.0, args.max_delay, seed=seed)
+        x = resident_input(hip, pool, T)
+        eng = corr.Engine(N, nf, chunk, corr.FORM_FP16)
+        eng.calibrate(x.ptr, T * L, P, stream=sp)
+        acc = hip.Buffer(B * nf * eng.integrator_bytes())
+        out = hip.Buffer(B * nf * 8)
+        subs.append(dict(N=N, nf=nf, T=T, L=L, B=B, pool=pool, x=x, eng=eng, acc=acc, out=out,
+                         tile=eng.xtile(), chunk=chunk))
+    stream.synchronize()
+    batch = corr.Batch([s["eng"] for s in subs])
+    items = [corr.Batch.item(s["x"].ptr, s["T"] * s["L"], s["T"], s["acc"].ptr, d_out=s["out"].ptr,
+                             dump_frames=s["T"]) for s in subs]
+
+    def sequential():
+        for s in subs:
+            hip.check(hip.rt().hipMemsetAsync(s["acc"].ptr, 0, s["acc"].nbytes, sp), "hipMemsetAsync")
+            s["eng"].integrate(s["x"].ptr, s["T"] * s["L"], s["T"], s["acc"].ptr, stream=sp)
+            s["eng"].dump(s["acc"].ptr, s["T"], s["out"].ptr, stream=sp)
+
+    def timed(fn, steps):
+        for _ in range(2):
+            fn()
+        stream.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        stream.synchronize()
+        return (time.perf_counter() - t0) / steps * 1e3
+
+    ms_seq = timed(sequential, args.batch_steps)
+    seq_out = [s["out"].to_numpy(np.uint32, (s["B"] * s["nf"] * 2,)).copy() for s in subs]
+    ms_direct = timed(lambda: batch.run(items, graph=False, stream=sp), args.batch_steps)
+    ms_graph = timed(lambda: batch.run(items, graph=True, stream=sp), args.batch_steps)
+    same = all(np.array_equal(s["out"].to_numpy(np.uint32, (s["B"] * s["nf"] * 2,)), o)
+               for s, o in zip(subs, seq_out))
+    samples = sum(s["N"] * s["T"] * s["L"] for s in subs)
+    inputs = sum(s["N"] for s in subs)
+    rate = samples / (ms_graph * 1e-3)
+    rep = []
+    for s in subs:
+        d = {"n_station": s["N"], "n_freq": s["nf"], "spectra": s["T"], "frames_per_launch": s["chunk"],
+             "channels_per_workgroup": s["tile"][0], "spectra_tile": s["tile"][1],
+             "integration_ms_realtime": round(s["T"] * s["L"] / 800e6 * 1e3, 3)}
+        if fp64:
+            V = s["out"].to_numpy(np.complex64, (s["B"], s["nf"]))
+            d["rel_rms_vs_fp64"] = rel_rms(V, fp64_visibilities(s["pool"]))
+        rep.append(d)
+    batch.close()
+    for s in subs:
+        s["eng"].close()
+        for k in ("x", "acc", "out"):
+            s[k].free()
+    stream.destroy()
+    return {"workload": "3 sub-arrays (20 in nf 8192 512 spectra; 24 in nf 16384 2048; 4 in nf 16384 8192), "
+                        "FP16 form, one dump of each per run",
+            "ms_per_run_graph": round(ms_graph, 4), "ms_per_run_direct": round(ms_direct, 4),
+            "ms_sequential": round(ms_seq, 4), "speedup_vs_sequential": round(ms_seq / ms_graph, 3),
+            "graph_equals_sequential": bool(same),
+            "Gsamples_per_s_total": round(rate / 1e9, 3),
+            "realtime_factor": round(rate / (inputs * 800e6), 2),
+            "subarrays": rep}
+
+
 def measured_traffic(N, nf, chunk, form):
     """HBM bytes per launch from the committed rocprofv3 PMC summary of this
     workload (FETCH_SIZE and WRITE_SIZE, calibrated on a copy of known size);
@@ -147,31 +282,34 @@ def main():
     corr = pkg.corr
     corr.load()
     hip = importlib.import_module("21cma_abf_correlator_amd.hip")
+    import numpy as np
 
-    N, nf, T, chunk = args.n_station, args.n_freq, args.frames, args.chunk
+    N, nf, T = args.n_station, args.n_freq, args.frames
     L = 2 * nf
     B = N * (N + 1) // 2
-    assert T % chunk == 0
     forms = [f.strip() for f in args.forms.split(",") if f.strip()]
     assert forms and all(f in FORMS for f in forms), args.forms
+
+    def chunk_of(name):
+        c = args.chunk or min(T, 512 if name == "fp32" else 2048)
+        assert T % c == 0, (T, c)
+        return c
+
     # synthetic input: a pool of distinct frames per input, tiled over the
     # integration in HBM (the throughput does not depend on the sample values)
     P = min(64, T)
     assert T % P == 0
     pool_host = gen_pool(N, P, L, args.rho, 16.0, args.max_delay)
-    pool = hip.Buffer.from_array(pool_host)
-    x = hip.Buffer(N * T * L)
-    for i in range(N):
-        for c in range(T // P):
-            hip.copy_d2d(x.ptr + (i * T + c * P) * L, pool.ptr + i * P * L, P * L)
-    pool.free()
+    x = resident_input(hip, pool_host, T)
     stream = hip.Stream()
     sp = stream.handle
-    n_chunks = T // chunk
     out = hip.Buffer(B * nf * 8)
+    region = ProfilerRegion()
 
     def run_form(name):
         form = FORMS[name]
+        chunk = chunk_of(name)
+        n_chunks = T // chunk
         eng = corr.Engine(N, nf, chunk, form)
         xtile = eng.xtile() if form != corr.FORM_FP32 else None
         acc = hip.Buffer(B * nf * eng.integrator_bytes())
@@ -182,45 +320,60 @@ def main():
             stream.synchronize()
             cal_ms = (time.perf_counter() - t0) * 1e3
 
-        def step(events=None):
+        def step():
+            # the whole integration in one call: the X-engine of each chunk runs
+            # on the engine's second stream beside the next chunk's F-engine
+            hip.check(hip.rt().hipMemsetAsync(acc.ptr, 0, acc.nbytes, sp), "hipMemsetAsync")
+            eng.integrate(x.ptr, T * L, T, acc.ptr, stream=sp)
+            eng.dump(acc.ptr, T, out.ptr, stream=sp)
+
+        def step_serial():
+            # the same work as F-engine, X-engine per chunk on one stream
             hip.check(hip.rt().hipMemsetAsync(acc.ptr, 0, acc.nbytes, sp), "hipMemsetAsync")
             for c in range(n_chunks):
-                ptr = x.ptr + c * chunk * L
-                if events is not None:
-                    events["f"].append((hip.Event(), hip.Event()))
-                    events["f"][-1][0].record(stream)
-                eng.fengine(ptr, T * L, chunk, stream=sp)
-                if events is not None:
-                    events["f"][-1][1].record(stream)
-                    events["x"].append((hip.Event(), hip.Event()))
-                    events["x"][-1][0].record(stream)
+                eng.fengine(x.ptr + c * chunk * L, T * L, chunk, stream=sp)
                 eng.xengine(acc.ptr, chunk, stream=sp)
-                if events is not None:
-                    events["x"][-1][1].record(stream)
-            if events is not None:
-                events["d"].append((hip.Event(), hip.Event()))
-                events["d"][-1][0].record(stream)
             eng.dump(acc.ptr, T, out.ptr, stream=sp)
-            if events is not None:
-                events["d"][-1][1].record(stream)
+
+        def timed(fn, steps):
+            stream.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                fn()
+            stream.synchronize()
+            return (time.perf_counter() - t0) / steps
 
         for _ in range(args.warmup):
+            step_serial()
             step()
-        stream.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            step()
-        stream.synchronize()
-        wall = time.perf_counter() - t0
-        import numpy as np
+        serial_s = timed(step_serial, max(1, min(args.steps, 3)))
+        vis_serial = out.to_numpy(np.complex64, (B, nf))
+        wall = timed(step, args.steps) * args.steps
         vis = out.to_numpy(np.complex64, (B, nf))
-        # per-kernel durations from HIP events on the launch stream (separate
-        # pass, so the timed loop above carries no event overhead)
+        assert np.array_equal(vis.view(np.uint32), vis_serial.view(np.uint32)), \
+            "overlapped and serial integrations differ"
+        # Per-kernel durations from HIP events on the launch stream, each kernel
+        # timed over back-to-back launches (the chunks of one integration), so
+        # the event records add no gap between launches of the same kernel.
+        # This pass alone is the profiler region: the trace of a profiled run
+        # holds exactly the launches timed here.
         ev = {"f": [], "x": [], "d": []}
-        for _ in range(max(1, min(args.steps, 3))):
-            step(ev)
-        stream.synchronize()
-        ms = {k: sum(a.elapsed_ms(b) for a, b in v) / len(v) for k, v in ev.items()}
+
+        def span(key, n, launch):
+            a, b = hip.Event(), hip.Event()
+            a.record(stream)
+            for c in range(n):
+                launch(c)
+            b.record(stream)
+            ev[key].append((a, b, n))
+
+        with region:
+            for _ in range(max(1, min(args.steps, 3))):
+                span("f", n_chunks, lambda c: eng.fengine(x.ptr + c * chunk * L, T * L, chunk, stream=sp))
+                span("x", n_chunks, lambda c: eng.xengine(acc.ptr, chunk, stream=sp))
+                span("d", 1, lambda c: eng.dump(acc.ptr, T, out.ptr, stream=sp))
+            stream.synchronize()
+        ms = {k: sum(a.elapsed_ms(b) for a, b, _ in v) / sum(n for _, _, n in v) for k, v in ev.items()}
         eng.close()
         acc.free()
         f_bytes = N * chunk * L + N * chunk * nf * SPEC_BYTES[name]   # int8 in + spectra out
@@ -242,10 +395,13 @@ def main():
                                 "frac_hbm": round(x_gbs / HBM_PEAK_GBS, 4)},
             f"dump_{name}": {"avg_ms": round(ms["d"], 4)},
         }
+        kern[f"fengine_{name}"]["frames_per_launch"] = chunk
+        kern[f"xengine_{name}"]["frames_per_launch"] = chunk
         if xtile is not None:
             kern[f"xengine_{name}"]["channels_per_workgroup"] = xtile[0]
             kern[f"xengine_{name}"]["spectra_tile"] = xtile[1]
-        return {"ms_per_step": wall / args.steps * 1e3,
+        return {"chunk": chunk, "ms_per_step": wall / args.steps * 1e3,
+                "ms_per_step_serial": serial_s * 1e3,
                 "value": 2 * nf * T / (wall / args.steps) / 1e6,
                 "ms": ms, "kernels": kern, "vis": vis, "calibrate_ms": cal_ms,
                 "f_gbs": f_gbs, "f_bytes": f_bytes, "x_tflops": x_tflops, "peak": peak,
@@ -262,6 +418,8 @@ def main():
     per_form = {}
     for name, r in res.items():
         d = {"value": round(r["value"], 2), "ms_per_step": round(r["ms_per_step"], 4),
+             "ms_per_step_serial": round(r["ms_per_step_serial"], 4),
+             "overlap_speedup": round(r["ms_per_step_serial"] / r["ms_per_step"], 3),
              "kernels": r["kernels"]}
         if r["calibrate_ms"] is not None:
             d["calibrate_ms"] = round(r["calibrate_ms"], 2)
@@ -271,11 +429,13 @@ def main():
                 d["rel_rms_vs_fp64_cross"] = rel_rms(r["vis"], ref, cross)
         if name != "fp32" and "fp32" in res:
             d["rel_rms_vs_fp32_form"] = rel_rms(r["vis"], res["fp32"]["vis"].astype(np.complex128))
-            d["xengine_speedup_vs_fp32"] = round(res["fp32"]["ms"]["x"] / r["ms"]["x"], 3)
+            # per frame: the forms may launch different frame counts
+            f32 = res["fp32"]
+            d["xengine_speedup_vs_fp32"] = round((f32["ms"]["x"] / f32["chunk"]) / (r["ms"]["x"] / r["chunk"]), 3)
             d["step_speedup_vs_fp32"] = round(res["fp32"]["ms_per_step"] / r["ms_per_step"], 3)
         per_form[name] = d
 
-    traffic = measured_traffic(N, nf, chunk, head)
+    traffic = measured_traffic(N, nf, h["chunk"], head)
     if h["ms"]["f"] >= h["ms"]["x"]:
         roof = {"bound": "hbm", "achieved": round(h["f_gbs"], 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(h["f_gbs"] / HBM_PEAK_GBS, 4),
@@ -294,9 +454,11 @@ def main():
     ms_per_step = h["ms_per_step"]
     kernels = h["kernels"]
 
+    batch = None if args.no_batch else batch_c5(corr, hip, args, not args.no_fp64)
+
     cpu = None
     if args.cpu_seconds > 0:
-        cpu = cpu_baseline(N, nf, args.cpu_seconds, chunk)
+        cpu = cpu_baseline(N, nf, args.cpu_seconds, h["chunk"])
 
     out = {
         "metric": METRIC,
@@ -313,13 +475,14 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"FX integration, {N} int8 inputs, n_freq {nf}, {T} spectra, "
                                f"{head} form, inputs resident in HBM",
-                   "n_station": N, "n_freq": nf, "spectra": T, "chunk_frames": chunk,
+                   "n_station": N, "n_freq": nf, "spectra": T, "chunk_frames": h["chunk"],
                    "form": head},
         "roofline": roof,
         "kernels": kernels,
         "forms": per_form,
         "realtime_Msps_per_input": 800.0,
         "cpu_baseline": cpu,
+        "batch": batch,
     }
     print(json.dumps(out))
 

@@ -63,6 +63,54 @@ corr_status corrx_fengine(void *engine, const int8_t *d_in, size_t station_pitch
    A zeroed integrator starts an integration. */
 corr_status corrx_xengine(void *engine, void *d_acc, int32_t frames, void *stream);
 
+/* F- and X-engine over `frames` frames (any count; frame f of station s at
+   d_in + s*station_pitch + f*2*n_freq, mask bits as corrx_fengine), added into
+   d_acc as corrx_xengine adds: the same bytes as corrx_fengine + corrx_xengine
+   over successive runs of at most chunk_frames frames. In the FP16 and INT8
+   forms with more than one run, the X-engine runs on the engine's own stream
+   over a second spectrum buffer, so that it overlaps the next run's F-engine;
+   the FP32 form (whose X-engine is VALU-bound like the F-engine) and single
+   runs go in turn on `stream`. Everything is ordered after the work already
+   enqueued on `stream`, and `stream` waits for the last X-engine before the
+   call returns. The first overlapped call allocates the second spectrum buffer
+   (as large as the first).
+   Replaces the per-chunk loop of reference kernels.cu:810-845 for callers that
+   hold a whole integration in device memory. */
+corr_status corrx_integrate(void *engine, const int8_t *d_in, size_t station_pitch,
+                            const uint8_t *d_mask, size_t mask_pitch, int32_t mask_bit0,
+                            int64_t frames, void *d_acc, void *stream);
+
+/* Batches: one dump of several independent sub-arrays (engines of any shapes
+   and forms) at once. No counterpart in the reference, which runs one array per
+   context; this is the batched form of corrx_integrate + corrx_dump.
+
+   Per engine, a run does, on a branch stream of its own forked from `stream`:
+   zero d_acc (flag CORRX_BATCH_ZERO; a fill kernel), corrx_integrate over
+   `frames` frames, and corrx_dump of d_acc as `dump_frames` frames into d_out
+   (flag CORRX_BATCH_DUMP); `stream` waits for every branch. With use_graph != 0
+   the run is captured into a hipGraph on first use and replayed while the
+   items stay byte-identical (other items: a new capture). The engines must be
+   distinct, on the current device, and outlive the batch. */
+#define CORRX_BATCH_ZERO 1
+#define CORRX_BATCH_DUMP 2
+
+typedef struct corrx_batch_item {
+    const int8_t *d_in;      /* as corrx_integrate */
+    size_t station_pitch;
+    const uint8_t *d_mask;   /* NULL: every frame present */
+    size_t mask_pitch;
+    int32_t mask_bit0;
+    int32_t flags;           /* CORRX_BATCH_ZERO | CORRX_BATCH_DUMP */
+    int64_t frames;          /* frames integrated in this run (>= 0) */
+    void *d_acc;             /* the engine's integrator, [B][n_freq] of its element */
+    uint64_t dump_frames;    /* nominal frame count of the dump (> 0 with CORRX_BATCH_DUMP) */
+    void *d_out;             /* complex64 [B][n_freq] (with CORRX_BATCH_DUMP) */
+} corrx_batch_item;          /* 72 bytes on x86-64 */
+
+corr_status corrx_batch_create(void *const *engines, int32_t n_engines, void **out_batch);
+corr_status corrx_batch_destroy(void *batch);
+corr_status corrx_batch_run(void *batch, const corrx_batch_item *items, int32_t use_graph, void *stream);
+
 /* Bytes of one integrator element (per baseline and channel) of the engine's form. */
 corr_status corrx_integrator_bytes(void *engine, size_t *bytes);
 

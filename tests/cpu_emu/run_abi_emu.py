@@ -78,6 +78,11 @@ def forms(L, form, N, nf, T, chunk, cal_frames):
         h = h.transpose(0, 2, 1, 3, 4).reshape(N, n, nf, 2)
         Yq[:, f0:f0 + n] = h[..., 0] + 1j * h[..., 1]
         e.xengine(ptr(integ), n)
+    # the two-buffer pipeline over the whole integration adds the same runs in
+    # the same order: the integrator must match byte for byte
+    integ2 = np.zeros_like(integ)
+    e.integrate(ptr(x), pitch, T, ptr(integ2), ptr(mask), mask_pitch, 0)
+    same = integ2.tobytes() == integ.tobytes()
     out = np.zeros(B * nf, dtype=np.complex64)
     e.dump(ptr(integ), T, ptr(out))
     e.close()
@@ -93,10 +98,33 @@ def forms(L, form, N, nf, T, chunk, cal_frames):
     missing_zero = not np.any(Yq[1, 3])
     e64 = rel_rms(V, ref)
     bound = 1e-3 if form == corr.FORM_FP16 else 0.3
-    ok = ok and missing_zero and e64 < bound and np.all(gains > 0)
+    ok = ok and missing_zero and e64 < bound and np.all(gains > 0) and same
     print(f"form {form} N={N} nf={nf} T={T} chunk={chunk}: vs oracle on engine spectra {eq:.3e}, "
-          f"vs FP64 {e64:.3e}, missing frame zero {missing_zero}", flush=True)
+          f"vs FP64 {e64:.3e}, missing frame zero {missing_zero}, integrate == loop {same}", flush=True)
     return ok
+
+
+def integrate_fp32(L, N, nf, T, chunk):
+    """FP32 form: corrx_integrate against the F-engine + X-engine loop (same
+    bytes) and against the FP64 oracle."""
+    x = oracle.gen_int8(77, N, T, 2 * nf, 0.3, max_delay=4)
+    e = corr.Engine(N, nf, chunk, corr.FORM_FP32, lib=L)
+    pitch = T * 2 * nf
+    B = N * (N + 1) // 2
+    a = np.zeros(B * nf, dtype=np.complex64)
+    for f0 in range(0, T, chunk):
+        n = min(chunk, T - f0)
+        e.fengine(ptr(x) + f0 * 2 * nf, pitch, n)
+        e.xengine(ptr(a), n)
+    b = np.zeros_like(a)
+    e.integrate(ptr(x), pitch, T, ptr(b))
+    e.integrate(ptr(x), pitch, 0, ptr(b))  # no frames: no work
+    e.close()
+    same = a.tobytes() == b.tobytes()
+    err = rel_rms(b.reshape(B, nf) / T, oracle.fx_f64(x))
+    print(f"integrate FP32 N={N} nf={nf} T={T} chunk={chunk}: integrate == loop {same}, vs FP64 {err:.3e}",
+          flush=True)
+    return same and err < 1e-5
 
 
 def main():
@@ -105,6 +133,7 @@ def main():
     ok = golden(L, 5, 4096, 37, 16, 3) and ok
     ok = forms(L, corr.FORM_FP16, 5, 512, 70, 32, 20) and ok
     ok = forms(L, corr.FORM_INT8, 6, 512, 70, 32, 70) and ok
+    ok = integrate_fp32(L, 3, 512, 45, 8) and ok
     print("OK" if ok else "FAILED")
     return 0 if ok else 1
 

@@ -144,6 +144,16 @@ def test_null_handles_rejected(corr):
     assert lib.corrx_calibrate(None, None, 0, None, 0, 0, 1, None) == corr.ERR_ARG
     assert lib.corrx_set_gains(None, None, None) == corr.ERR_ARG
     assert lib.corrx_gains(None, ctypes.byref(ctypes.c_void_p())) == corr.ERR_ARG
+    assert lib.corrx_integrate(None, None, 0, None, 0, 0, 1, None, None) == corr.ERR_ARG
+    b = ctypes.c_void_p(0x1)
+    assert lib.corrx_batch_create(None, 1, ctypes.byref(b)) == corr.ERR_ARG and not b.value
+    two = (ctypes.c_void_p * 2)(None, None)
+    b = ctypes.c_void_p(0x1)
+    assert lib.corrx_batch_create(two, 2, ctypes.byref(b)) == corr.ERR_ARG and not b.value  # NULL engines
+    assert lib.corrx_batch_create(two, 0, ctypes.byref(b)) == corr.ERR_ARG
+    assert lib.corrx_batch_run(None, None, 1, None) == corr.ERR_ARG
+    assert lib.corrx_batch_destroy(None) == corr.OK
+    assert ctypes.sizeof(corr.corrx_batch_item) == 72
 
 
 @pytest.mark.parametrize("shape,status", [
