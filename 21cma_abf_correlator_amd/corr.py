@@ -387,7 +387,7 @@ class Engine:
         sh = corrx_shape(n_station, n_freq, chunk_frames, form)
         h = ctypes.c_void_p()
         check(self._lib.corrx_engine_create(ctypes.byref(sh), ctypes.byref(h)),
-              "corrx_engine_create")
+              "corrx_engine_create", self._lib)
         self.h = h.value
         self.n_station, self.n_freq, self.chunk_frames, self.form = \
             n_station, n_freq, chunk_frames, form
@@ -406,55 +406,55 @@ class Engine:
     def fengine(self, d_in, station_pitch, frames, d_mask=None, mask_pitch=0, mask_bit0=0,
                 stream=None):
         check(self._lib.corrx_fengine(self.h, d_in, station_pitch, d_mask, mask_pitch, mask_bit0,
-                                      frames, stream), "corrx_fengine")
+                                      frames, stream), "corrx_fengine", self._lib)
 
     def xengine(self, d_acc, frames, stream=None):
-        check(self._lib.corrx_xengine(self.h, d_acc, frames, stream), "corrx_xengine")
+        check(self._lib.corrx_xengine(self.h, d_acc, frames, stream), "corrx_xengine", self._lib)
 
     def integrate(self, d_in, station_pitch, frames, d_acc, d_mask=None, mask_pitch=0, mask_bit0=0,
                   stream=None):
         """F- and X-engine over `frames` frames (any count) into d_acc, the
         X-engine of each run of chunk_frames overlapping the next F-engine."""
         check(self._lib.corrx_integrate(self.h, d_in, station_pitch, d_mask, mask_pitch, mask_bit0,
-                                        frames, d_acc, stream), "corrx_integrate")
+                                        frames, d_acc, stream), "corrx_integrate", self._lib)
 
     def spectra(self):
         p = ctypes.c_void_p()
         s = ctypes.c_size_t()
-        check(self._lib.corrx_spectra(self.h, ctypes.byref(p), ctypes.byref(s)), "corrx_spectra")
+        check(self._lib.corrx_spectra(self.h, ctypes.byref(p), ctypes.byref(s)), "corrx_spectra", self._lib)
         return p.value, s.value
 
     def integrator_bytes(self):
         """Bytes of one integrator element: 8 (FP32), 16 (FP16), 8 (INT8)."""
         n = ctypes.c_size_t()
-        check(self._lib.corrx_integrator_bytes(self.h, ctypes.byref(n)), "corrx_integrator_bytes")
+        check(self._lib.corrx_integrator_bytes(self.h, ctypes.byref(n)), "corrx_integrator_bytes", self._lib)
         return n.value
 
     def dump(self, d_acc, frames, d_out, stream=None):
-        check(self._lib.corrx_dump(self.h, d_acc, frames, d_out, stream), "corrx_dump")
+        check(self._lib.corrx_dump(self.h, d_acc, frames, d_out, stream), "corrx_dump", self._lib)
 
     def calibrate(self, d_in, station_pitch, frames, d_mask=None, mask_pitch=0, mask_bit0=0,
                   stream=None):
         check(self._lib.corrx_calibrate(self.h, d_in, station_pitch, d_mask, mask_pitch,
-                                        mask_bit0, frames, stream), "corrx_calibrate")
+                                        mask_bit0, frames, stream), "corrx_calibrate", self._lib)
 
     def set_gains(self, d_gains, stream=None):
-        check(self._lib.corrx_set_gains(self.h, d_gains, stream), "corrx_set_gains")
+        check(self._lib.corrx_set_gains(self.h, d_gains, stream), "corrx_set_gains", self._lib)
 
     def gains(self):
         p = ctypes.c_void_p()
-        check(self._lib.corrx_gains(self.h, ctypes.byref(p)), "corrx_gains")
+        check(self._lib.corrx_gains(self.h, ctypes.byref(p)), "corrx_gains", self._lib)
         return p.value
 
     def xtile(self):
         """(channels per MFMA X-engine workgroup, channel tile of the spectrum order)."""
         c = ctypes.c_int32()
         t = ctypes.c_int32()
-        check(self._lib.corrx_get_xtile(self.h, ctypes.byref(c), ctypes.byref(t)), "corrx_get_xtile")
+        check(self._lib.corrx_get_xtile(self.h, ctypes.byref(c), ctypes.byref(t)), "corrx_get_xtile", self._lib)
         return c.value, t.value
 
     def set_xtile(self, channels, spectra_tile):
-        check(self._lib.corrx_set_xtile(self.h, channels, spectra_tile), "corrx_set_xtile")
+        check(self._lib.corrx_set_xtile(self.h, channels, spectra_tile), "corrx_set_xtile", self._lib)
 
 
 class Batch:
@@ -467,7 +467,7 @@ class Batch:
         self.engines = list(engines)
         arr = (ctypes.c_void_p * len(self.engines))(*[e.h for e in self.engines])
         h = ctypes.c_void_p()
-        check(self._lib.corrx_batch_create(arr, len(self.engines), ctypes.byref(h)), "corrx_batch_create")
+        check(self._lib.corrx_batch_create(arr, len(self.engines), ctypes.byref(h)), "corrx_batch_create", self._lib)
         self.h = h.value
 
     def close(self):
@@ -490,7 +490,7 @@ class Batch:
 
     def run(self, items, graph=True, stream=None):
         arr = (corrx_batch_item * len(items))(*items)
-        check(self._lib.corrx_batch_run(self.h, arr, 1 if graph else 0, stream), "corrx_batch_run")
+        check(self._lib.corrx_batch_run(self.h, arr, 1 if graph else 0, stream), "corrx_batch_run", self._lib)
 
 
 def scale(d_acc, n_complex, frames, stream=None):

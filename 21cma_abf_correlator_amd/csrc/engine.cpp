@@ -9,6 +9,7 @@
 
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -163,24 +164,29 @@ corr_status Engine::init() {
 
 namespace {
 struct TileKey {
-    int form, n_station, n_freq, device;
+    int form, n_station, n_freq, frames, device;
     bool operator<(const TileKey &o) const {
-        return std::tie(form, n_station, n_freq, device) < std::tie(o.form, o.n_station, o.n_freq, o.device);
+        return std::tie(form, n_station, n_freq, frames, device) <
+               std::tie(o.form, o.n_station, o.n_freq, o.frames, o.device);
     }
 };
 std::mutex g_tile_mu;
 std::map<TileKey, std::pair<int, int>> g_tile_cache;  // (channels per workgroup, spectrum tile)
+constexpr int kTuneRounds = 5;
+constexpr int kTuneFrames = 2048;
 }  // namespace
 
 // The tile autotuner. Two choices interact: the channel width of the
 // X-engine's workgroups (16, or 32 up to 24 inputs) and the channel tile of
 // the spectrum order (that width, whose runs the X-engine reads contiguously,
 // or n_freq, the plain order the F-engine writes fastest). Every combination
-// that fits the shape is timed on the engine's own buffers (F-engine over
-// min(chunk_frames, 512) frames of zeros, X-engine over them: the call size,
-// where the steady state of both kernels shows; one warm-up and three timed
-// rounds); the fastest wins. The choice is cached per (form, inputs, n_freq,
-// device).
+// that fits the shape is timed on the engine's own buffers: F-engine over
+// min(chunk_frames, kTuneFrames) frames of zeros and X-engine over them, the
+// launch size itself (at fewer frames the X-engine's fixed cost per launch
+// hides what the spectrum order changes); one warm-up round, then the fastest
+// of kTuneRounds rounds timed one by one, so that one disturbed round does not
+// decide. The fastest candidate wins. The choice is cached per (form, inputs,
+// n_freq, frames timed, device).
 corr_status Engine::tune_xtile() {
     const int N = shape_.n_station, nf = shape_.n_freq;
     std::vector<std::pair<int, int>> cand;
@@ -192,10 +198,26 @@ corr_status Engine::tune_xtile() {
     if (cand.empty()) return failf(CORR_ERR_NOT_IMPL, "xengine: no MFMA tile fits %d inputs", N);
     x_tile_ = cand[0].first;
     spec_tile_ = cand[0].second;
+    // CORRX_XTILE=<channels>,<spectrum tile> (tile 0: n_freq, the plain order)
+    // pins the choice wherever it fits the shape, with no timing at creation;
+    // where it does not fit, the tuner runs.
+    if (const char *env = std::getenv("CORRX_XTILE")) {
+        int ch = 0, tl = -1;
+        if (std::sscanf(env, "%d,%d", &ch, &tl) == 2) {
+            if (tl == 0) tl = nf;
+            for (const auto &c : cand)
+                if (c.first == ch && c.second == tl) {
+                    x_tile_ = ch;
+                    spec_tile_ = tl;
+                    return CORR_OK;
+                }
+        }
+    }
     int dev = 0;
     corr_status st;
     if ((st = hip_check(hipGetDevice(&dev), "hipGetDevice")) != CORR_OK) return st;
-    const TileKey key{(int)shape_.form, N, nf, dev};
+    const int frames = shape_.chunk_frames < kTuneFrames ? shape_.chunk_frames : kTuneFrames;
+    const TileKey key{(int)shape_.form, N, nf, frames, dev};
     {
         std::lock_guard<std::mutex> lk(g_tile_mu);
         auto it = g_tile_cache.find(key);
@@ -205,8 +227,10 @@ corr_status Engine::tune_xtile() {
             return CORR_OK;
         }
     }
-    const int frames = shape_.chunk_frames < 512 ? shape_.chunk_frames : 512;
-    const size_t in_pitch = (size_t)frames * 2u * (size_t)nf;
+    // One station's frames, read by every station (station pitch 0): the
+    // F-engine is VALU-bound, and the candidates differ in how the spectra are
+    // written and read, not in how the input is.
+    const size_t in_bytes = (size_t)frames * 2u * (size_t)nf;
     const size_t integ_bytes = (size_t)n_baseline_ * (size_t)nf * integrator_elem_bytes();
     void *integ = nullptr, *in = nullptr;
     hipStream_t s = nullptr;
@@ -214,15 +238,15 @@ corr_status Engine::tune_xtile() {
     float best_ms = 0.0f;
     std::pair<int, int> best = cand[0];
     if ((st = hip_check(hipMalloc(&integ, integ_bytes), "hipMalloc(tuning integrator)")) == CORR_OK &&
-        (st = hip_check(hipMalloc(&in, in_pitch * (size_t)N), "hipMalloc(tuning input)")) == CORR_OK &&
+        (st = hip_check(hipMalloc(&in, in_bytes), "hipMalloc(tuning input)")) == CORR_OK &&
         (st = hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate")) == CORR_OK &&
         (st = hip_check(hipEventCreate(&e0), "hipEventCreate")) == CORR_OK &&
         (st = hip_check(hipEventCreate(&e1), "hipEventCreate")) == CORR_OK &&
-        (st = hip_check(launch_fill(in, in_pitch * (size_t)N / 4, 0u, s), "fill_kernel(input)")) == CORR_OK &&
+        (st = hip_check(launch_fill(in, in_bytes / 4, 0u, s), "fill_kernel(input)")) == CORR_OK &&
         (st = hip_check(launch_fill(integ, integ_bytes / 4, 0u, s), "fill_kernel(integrator)")) == CORR_OK) {
         auto round = [&](const std::pair<int, int> &c) {
             corr_status r = hip_check(launch_fengine_form((int)shape_.form, log2_nf_, static_cast<const int8_t *>(in),
-                                                          in_pitch, nullptr, 0, d_tw_, d_wl_, d_gain_, d_spec_,
+                                                          0, nullptr, 0, d_tw_, d_wl_, d_gain_, d_spec_,
                                                           spectra_station_stride(), c.second, frames, N, s),
                                       "fengine_form_kernel");
             if (r != CORR_OK) return r;
@@ -231,16 +255,20 @@ corr_status Engine::tune_xtile() {
                              "xengine_mfma_kernel");
         };
         for (const auto &c : cand) {
-            if ((st = round(c)) != CORR_OK || (st = hip_check(hipEventRecord(e0, s), "hipEventRecord")) != CORR_OK)
-                break;
-            for (int r = 0; r < 3 && st == CORR_OK; ++r) st = round(c);
-            float ms = 0.0f;
-            if (st != CORR_OK || (st = hip_check(hipEventRecord(e1, s), "hipEventRecord")) != CORR_OK ||
-                (st = hip_check(hipEventSynchronize(e1), "hipEventSynchronize")) != CORR_OK ||
-                (st = hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime")) != CORR_OK)
-                break;
-            if (c == cand[0] || ms < best_ms) {
-                best_ms = ms;
+            if ((st = round(c)) != CORR_OK) break;  // warm-up
+            float cand_ms = 0.0f;
+            for (int r = 0; r < kTuneRounds && st == CORR_OK; ++r) {
+                float ms = 0.0f;
+                if ((st = hip_check(hipEventRecord(e0, s), "hipEventRecord")) != CORR_OK || (st = round(c)) != CORR_OK ||
+                    (st = hip_check(hipEventRecord(e1, s), "hipEventRecord")) != CORR_OK ||
+                    (st = hip_check(hipEventSynchronize(e1), "hipEventSynchronize")) != CORR_OK ||
+                    (st = hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime")) != CORR_OK)
+                    break;
+                if (r == 0 || ms < cand_ms) cand_ms = ms;
+            }
+            if (st != CORR_OK) break;
+            if (c == cand[0] || cand_ms < best_ms) {
+                best_ms = cand_ms;
                 best = c;
             }
         }

@@ -17,6 +17,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <type_traits>
 
 #include "forms.hpp"
 
@@ -267,22 +268,17 @@ CORR_HD CORR_FI void expand_validity(int f, int st, int frames, const uint8_t *m
     if (f < frames) valid[(size_t)st * valid_stride + (size_t)f] = mask_bit(mask, mask_pitch, bit0, f, st);
 }
 
-// Thread j's 16 input values of frame f of station st: z[j + r*N/16], r < 16,
-// as int8 pairs; zeros when the frame is missing (it is then never read, which
-// is bit for bit what a frame the host had zeroed gives).
+// Thread j's 16 input values of frame f of station st, z[j + r*N/16] for
+// r < 16, converted exactly from int8 pairs.
 template <int LOG2N>
-CORR_HD CORR_FI void load_frame(int j, const int8_t *in, size_t station_pitch, int f, int st, bool present,
-                                char2 (&raw)[16]) {
+CORR_HD CORR_FI void load_frame(int j, const int8_t *in, size_t station_pitch, int f, int st, float2 (&v)[16]) {
     constexpr int NT = (1 << LOG2N) / 16;
-    if (present) {
-        const char2 *src = reinterpret_cast<const char2 *>(in + (size_t)st * station_pitch +
-                                                           (size_t)f * (size_t)(2 << LOG2N));
+    const char2 *src = reinterpret_cast<const char2 *>(in + (size_t)st * station_pitch + (size_t)f * (size_t)(2 << LOG2N));
+    char2 raw[16];
 #pragma unroll
-        for (int r = 0; r < 16; ++r) raw[r] = src[(unsigned)(j + r * NT)];
-    } else {
+    for (int r = 0; r < 16; ++r) raw[r] = src[(unsigned)(j + r * NT)];
 #pragma unroll
-        for (int r = 0; r < 16; ++r) raw[r].x = raw[r].y = 0;
-    }
+    for (int r = 0; r < 16; ++r) v[r] = make_float2((float)raw[r].x, (float)raw[r].y);
 }
 
 // Frame f of station st is present: `valid` is NULL (every frame present) or
@@ -309,13 +305,9 @@ CORR_HD CORR_FI void fft_from_values(int j, float2 (&v)[16], float2 *lds, Sync &
 // Thread j of N/16; on return `lds` holds Z after a final barrier.
 template <int LOG2N, class Sync>
 CORR_HD CORR_FI void complex_fft_block(int j, int f, int st, float2 *lds, Sync &sync, const int8_t *in,
-                                      size_t station_pitch, const int32_t *valid, size_t valid_stride,
-                                      const float2 *tw) {
-    char2 raw[16];
-    load_frame<LOG2N>(j, in, station_pitch, f, st, frame_present(valid, valid_stride, f, st), raw);
+                                      size_t station_pitch, const float2 *tw) {
     float2 v[16];
-#pragma unroll
-    for (int r = 0; r < 16; ++r) v[r] = make_float2((float)raw[r].x, (float)raw[r].y);
+    load_frame<LOG2N>(j, in, station_pitch, f, st, v);
     fft_from_values<LOG2N>(j, v, lds, sync, tw);
 }
 
@@ -349,26 +341,27 @@ CORR_HD CORR_FI void r2c_pair(const float2 *lds, float2 w, int k, float2 &xk, fl
 // thread j taking k = j + q*N/16 (q < 8) with its mirror N - k, thread 0 also
 // the self-mirrored bin N/2 (bin 0's mirror is the Nyquist bin, never formed).
 // W^k = W^j W^(q N/16) with W^(q N/16) = exp(-i pi q/16), a constant: one
-// table read per thread. emit(k, X) stores one bin.
+// table read per thread. The emitter stores them: emit.pair(q, X[k], X[N-k])
+// for k = j + q*N/16 != 0, emit.at(k, X[k]) for bins 0 and N/2.
 template <int N, class Emit>
 CORR_HD CORR_FI void r2c_epilogue(int j, const float2 *lds, const float2 *wl, Emit &&emit) {
     constexpr int NT = N / 16;
     const float2 wj = wl[(unsigned)j];
+    emit.begin();
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
         const int k = j + q * NT;
         if (k == 0) {
-            emit(0, r2c_bin<N>(lds, wl, 0));
+            emit.at(0, r2c_bin<N>(lds, wl, 0));
         } else {
             // exp(-i pi q/16) = cos(2 pi q/32) - i sin(2 pi q/32)
             const float2 w = q == 0 ? wj : cmul(wj, make_float2(cos32(q), -cos32((q + 24) & 31)));
             float2 xk, xm;
             r2c_pair<N>(lds, w, k, xk, xm);
-            emit(k, xk);
-            emit(N - k, xm);
+            emit.pair(q, xk, xm);
         }
     }
-    if (j == 0) emit(N / 2, r2c_bin<N>(lds, wl, N / 2));
+    if (j == 0) emit.at(N / 2, r2c_bin<N>(lds, wl, N / 2));
 }
 
 // Whole F-engine workgroup, complex64 output [station][frame][N].
@@ -377,6 +370,28 @@ CORR_HD CORR_FI void fengine_frame(int j, int f, int st, float2 *lds, Sync &sync
                                    size_t station_pitch, const int32_t *valid, size_t valid_stride,
                                    const float2 *tw, const float2 *wl, Emit &&emit);
 
+// Emitters store the R2C bins of lane j: begin() before the first store,
+// at(k, X) bin k (bins 0 and N/2 only), pair(q, X[k], X[N-k]) for
+// k = j + q*N/16; zero_at / zero_pair store zero bins (a missing frame). Each bin's address is a base that
+// depends on q alone, the same for the whole workgroup, plus the lane's own
+// fixed offset: the base is scalar arithmetic, so a store costs no vector
+// address arithmetic. The mirror bin N - j - q*N/16 is addressed as
+// base(N - (q+1)*N/16) + offset(N/16 - j).
+template <int N>
+struct Fp32Emit {
+    float2 *dst;  // the frame's first bin
+    int j;
+    CORR_HD CORR_FI void begin() {}
+    CORR_HD CORR_FI void at(int k, float2 x) const { dst[(unsigned)k] = x; }
+    CORR_HD CORR_FI void pair(int q, float2 xk, float2 xm) const {
+        constexpr int NT = N / 16;
+        (dst + q * NT)[(unsigned)j] = xk;
+        (dst + (N - (q + 1) * NT))[(unsigned)(NT - j)] = xm;
+    }
+    CORR_HD CORR_FI void zero_at(int k) const { at(k, make_float2(0.0f, 0.0f)); }
+    CORR_HD CORR_FI void zero_pair(int q) const { pair(q, make_float2(0.0f, 0.0f), make_float2(0.0f, 0.0f)); }
+};
+
 // Whole F-engine workgroup, complex64 output [station][frame][N].
 template <int LOG2N, class Sync>
 CORR_HD CORR_FI void fengine_fp32_block(int j, int f, int st, float2 *lds, Sync &sync, const int8_t *in,
@@ -384,10 +399,50 @@ CORR_HD CORR_FI void fengine_fp32_block(int j, int f, int st, float2 *lds, Sync 
                                        const float2 *tw, const float2 *wl, float2 *out,
                                        size_t out_station_stride) {
     constexpr int N = 1 << LOG2N;
-    float2 *dst = out + (size_t)st * out_station_stride + (size_t)f * (size_t)N;
-    fengine_frame<LOG2N>(j, f, st, lds, sync, in, station_pitch, valid, valid_stride, tw, wl,
-                         [&](int k, float2 x) { dst[(unsigned)k] = x; });
+    Fp32Emit<N> e{out + (size_t)st * out_station_stride + (size_t)f * (size_t)N, j};
+    fengine_frame<LOG2N>(j, f, st, lds, sync, in, station_pitch, valid, valid_stride, tw, wl, e);
 }
+
+// The form's emitter: bin k scaled by its gain and converted (forms.hpp), at
+// offset off(k) = (k >> shift) * tile_stride + k % 2^shift from the frame's
+// base. off() is additive over a multiple of the tile plus a remainder, which
+// the pair addressing needs: the tile is at most N/16 channels, or N.
+template <int N, int FORM>
+struct FormEmit {
+    using word_t = typename std::conditional<FORM == 1, uint32_t, uint16_t>::type;
+    word_t *dst;         // the frame's base in the station's tile-major block
+    const float *g;      // the station's gains
+    unsigned shift, mask, tile_stride;
+    unsigned ok = 0, om = 0;  // off(j), off(N/16 - j), set by begin()
+    int j;
+    CORR_HD CORR_FI unsigned off(unsigned k) const { return (k >> shift) * tile_stride + (k & mask); }
+    // The lane offsets are formed here, after the transform, so that they do
+    // not hold registers through it.
+    CORR_HD CORR_FI void begin() {
+        ok = off((unsigned)j);
+        om = off((unsigned)(N / 16 - j));
+    }
+    CORR_HD CORR_FI static word_t conv(float2 x, float gain) {
+        if constexpr (FORM == 1)
+            return forms::pack_fp16(x, gain);
+        else
+            return forms::pack_int8(x, gain);
+    }
+    CORR_HD CORR_FI void at(int k, float2 x) const { dst[off((unsigned)k)] = conv(x, g[(unsigned)k]); }
+    CORR_HD CORR_FI void pair(int q, float2 xk, float2 xm) const {
+        constexpr int NT = N / 16;
+        const unsigned bk = (unsigned)(q * NT), bm = (unsigned)(N - (q + 1) * NT);
+        (dst + off(bk))[ok] = conv(xk, (g + bk)[(unsigned)j]);
+        (dst + off(bm))[om] = conv(xm, (g + bm)[(unsigned)(NT - j)]);
+    }
+    // zero requantizes to the all-zero word in both forms
+    CORR_HD CORR_FI void zero_at(int k) const { dst[off((unsigned)k)] = 0; }
+    CORR_HD CORR_FI void zero_pair(int q) const {
+        constexpr int NT = N / 16;
+        (dst + off((unsigned)(q * NT)))[ok] = 0;
+        (dst + off((unsigned)(N - (q + 1) * NT)))[om] = 0;
+    }
+};
 
 // Whole F-engine workgroup for the FP16 (FORM 1) or int8 (FORM 2) form: bin k
 // of station st is scaled by gain[st*N + k] and converted (forms.hpp); output
@@ -400,21 +455,17 @@ CORR_HD CORR_FI void fengine_form_block(int j, int f, int st, float2 *lds, Sync 
                                        const float2 *tw, const float2 *wl, const float *gain,
                                        void *out, size_t out_station_stride, int tile_shift) {
     constexpr int N = 1 << LOG2N;
-    const float *g = gain + (size_t)st * N;
+    using Emit = FormEmit<N, FORM>;
     // bin k -> tile k >> shift at (tile * fcap + f) * T + k % T from the station's base
-    const unsigned tmask = (1u << tile_shift) - 1u;
-    const unsigned tile_stride = (unsigned)(out_station_stride / N) << tile_shift;  // fcap * T
-    const size_t at = (size_t)st * out_station_stride + ((size_t)f << tile_shift);
-    auto off = [&](int k) { return ((unsigned)k >> tile_shift) * tile_stride + ((unsigned)k & tmask); };
-    if constexpr (FORM == 1) {
-        uint32_t *dst = static_cast<uint32_t *>(out) + at;
-        fengine_frame<LOG2N>(j, f, st, lds, sync, in, station_pitch, valid, valid_stride, tw, wl,
-                             [&](int k, float2 x) { dst[off(k)] = forms::pack_fp16(x, g[(unsigned)k]); });
-    } else {
-        uint16_t *dst = static_cast<uint16_t *>(out) + at;
-        fengine_frame<LOG2N>(j, f, st, lds, sync, in, station_pitch, valid, valid_stride, tw, wl,
-                             [&](int k, float2 x) { dst[off(k)] = forms::pack_int8(x, g[(unsigned)k]); });
-    }
+    Emit e;
+    e.dst = static_cast<typename Emit::word_t *>(out) + (size_t)st * out_station_stride +
+            ((size_t)f << tile_shift);
+    e.g = gain + (size_t)st * N;
+    e.shift = (unsigned)tile_shift;
+    e.mask = (1u << tile_shift) - 1u;
+    e.tile_stride = (unsigned)(out_station_stride / N) << tile_shift;  // fcap * T
+    e.j = j;
+    fengine_frame<LOG2N>(j, f, st, lds, sync, in, station_pitch, valid, valid_stride, tw, wl, e);
 }
 
 // ---------------------------------------------------------------------------
@@ -584,23 +635,23 @@ CORR_HD CORR_FI void fengine_half_16384(int j, float2 (&v)[16], float2 *lds, Syn
     }
     // b[0] of lane 0 is Z[N/2] (read at index 0 of half 1); bin 0 pairs with
     // itself: X[0] = Re Z[0] + Im Z[0].
+    emit.begin();
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
         const int k = j + q * NT;
         const float2 w = q == 0 ? wj : cmul(wj, make_float2(cos32(q), -cos32((q + 24) & 31)));
         if (k == 0) {
-            emit(0, make_float2(a[0].x + a[0].y, 0.0f));
+            emit.at(0, make_float2(a[0].x + a[0].y, 0.0f));
         } else {
             const float er = 0.5f * (a[q].x + b[q].x), ei = 0.5f * (a[q].y - b[q].y);
             const float orr = 0.5f * (a[q].y + b[q].y), oi = -0.5f * (a[q].x - b[q].x);
             const float tr = w.x * orr - w.y * oi, ti = w.x * oi + w.y * orr;
-            emit(k, make_float2(er + tr, ei + ti));
-            emit(N - k, make_float2(er - tr, ti - ei));
+            emit.pair(q, make_float2(er + tr, ei + ti), make_float2(er - tr, ti - ei));
         }
     }
     if (j == 0) {
         // X[N/2] = (Z + conj Z)/2 + W^(N/2) (Z - conj Z)/(2i) with W^(N/2) = -i
-        emit(N / 2, make_float2(zh.x, -zh.y));
+        emit.at(N / 2, make_float2(zh.x, -zh.y));
     }
 }
 
@@ -610,22 +661,42 @@ CORR_HD constexpr int fengine_lds_elems() {
     return LOG2N == 14 ? HalfSizes<(1 << LOG2N)>::kLds : Sizes<(1 << LOG2N)>::kLds;
 }
 
+// Lane j's bins of a missing frame, whose spectrum is zero, in the order the
+// R2C epilogue emits them.
+template <int N, class Emit>
+CORR_HD CORR_FI void emit_zero_frame(int j, Emit &&emit) {
+    emit.begin();
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        if (j + q * (N / 16) == 0)
+            emit.zero_at(0);
+        else
+            emit.zero_pair(q);
+    }
+    if (j == 0) emit.zero_at(N / 2);
+}
+
 // One frame of one station through the F-engine: N = 16384 exchanges through
-// the half buffer, shorter transforms through the full one.
+// the half buffer, shorter transforms through the full one. A missing frame is
+// neither read nor transformed: its spectrum is written as zeros, the value the
+// transform of a zeroed frame has (up to the sign of a zero, which changes no
+// sum the X-engine forms). Presence is the same for the whole workgroup, so
+// the early return splits no barrier.
 template <int LOG2N, class Sync, class Emit>
 CORR_HD CORR_FI void fengine_frame(int j, int f, int st, float2 *lds, Sync &sync, const int8_t *in,
                                    size_t station_pitch, const int32_t *valid, size_t valid_stride,
                                    const float2 *tw, const float2 *wl, Emit &&emit) {
     constexpr int N = 1 << LOG2N;
+    if (!frame_present(valid, valid_stride, f, st)) {
+        emit_zero_frame<N>(j, emit);
+        return;
+    }
     if constexpr (LOG2N == 14) {
-        char2 raw[16];
-        load_frame<LOG2N>(j, in, station_pitch, f, st, frame_present(valid, valid_stride, f, st), raw);
         float2 v[16];
-#pragma unroll
-        for (int r = 0; r < 16; ++r) v[r] = make_float2((float)raw[r].x, (float)raw[r].y);
+        load_frame<LOG2N>(j, in, station_pitch, f, st, v);
         fengine_half_16384(j, v, lds, sync, tw, wl, emit);
     } else {
-        complex_fft_block<LOG2N>(j, f, st, lds, sync, in, station_pitch, valid, valid_stride, tw);
+        complex_fft_block<LOG2N>(j, f, st, lds, sync, in, station_pitch, tw);
         r2c_epilogue<N>(j, lds, wl, emit);
     }
 }

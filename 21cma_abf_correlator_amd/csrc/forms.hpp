@@ -89,7 +89,15 @@ CORR_HD inline float f16_bits_to_f32(uint16_t h) {
 }
 
 CORR_HD inline uint32_t pack_fp16(float2 y, float g) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    // as a two-half vector the compiler writes the high half in place
+    // (v_fma_mixhi_f16) instead of shifting and or-ing it in
+    typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+    const half2_t v = {(_Float16)(y.x * g), (_Float16)(y.y * g)};
+    return __builtin_bit_cast(uint32_t, v);
+#else
     return (uint32_t)f32_to_f16_bits(y.x * g) | ((uint32_t)f32_to_f16_bits(y.y * g) << 16);
+#endif
 }
 
 CORR_HD inline int quant_int8(float v) {
@@ -98,7 +106,17 @@ CORR_HD inline int quant_int8(float v) {
 }
 
 CORR_HD inline uint16_t pack_int8(float2 y, float g) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    // Clamping to +-7 before rounding gives the same integer (the bounds are
+    // integers). Adding 1.5 * 2^23 then rounds to nearest even, as rintf does,
+    // and leaves the integer's two's-complement low byte in the sum's low bits.
+    constexpr float kMagic = 12582912.0f;
+    const float a = fminf(fmaxf(y.x * g, -(float)kInt8Max), (float)kInt8Max) + kMagic;
+    const float b = fminf(fmaxf(y.y * g, -(float)kInt8Max), (float)kInt8Max) + kMagic;
+    return (uint16_t)((__builtin_bit_cast(uint32_t, a) & 0xFFu) | ((__builtin_bit_cast(uint32_t, b) & 0xFFu) << 8));
+#else
     return (uint16_t)((uint32_t)(quant_int8(y.x * g) & 0xFF) | ((uint32_t)(quant_int8(y.y * g) & 0xFF) << 8));
+#endif
 }
 
 // Spectrum buffer of the FP16 and int8 forms: channels in tiles of `tile` (16

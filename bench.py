@@ -67,6 +67,8 @@ def parse():
     p.add_argument("--no-batch", action="store_true",
                    help="skip the batched three-sub-array workload (one hipGraph per dump)")
     p.add_argument("--batch-steps", type=int, default=5)
+    p.add_argument("--no-abi", action="store_true",
+                   help="skip the end-to-end drop-in ABI leg (pinned staging -> host visibilities)")
     return p.parse_args()
 
 
@@ -254,6 +256,63 @@ def batch_c5(corr, hip, args, fp64):
             "Gsamples_per_s_total": round(rate / 1e9, 3),
             "realtime_factor": round(rate / (inputs * 800e6), 2),
             "subarrays": rep}
+
+
+def abi_end_to_end(corr, args, pool_host, fp64):
+    """The drop-in path end to end: frames in pinned staging (as the receiver
+    leaves them) -> corr_row_accumulate (H2D + F + X, FP32 form) per slot ->
+    corr_row_end -> one corr_rows_fetch of the row stack to host memory.
+    Geometry of the reference's 24-input nf 16384 YAML: 256 frames per slot,
+    11 slots, rows of 2048 frames. The staging is written once before timing;
+    each released slot gets its mask bits back (the library clears them on
+    release, the receiver sets them). PCIe-inclusive: not the headline value."""
+    import numpy as np
+    N, P, L = pool_host.shape
+    nf = L // 2
+    fps, slots, n_cum, rows = 256, 11, 2048, 4
+    cfg = corr.make_config(N, nf, fps, fps, block_slots=slots, rows_per_fetch=rows)
+    with corr.Correlator(cfg) as c:
+        B = c.n_baseline
+        for g in range(N):
+            for sl in range(slots):
+                c.slot_frames(g, sl)[:] = pool_host[g, (sl * fps + np.arange(fps)) % P]
+        masks = [[c.slot_mask(g, sl) for sl in range(slots)] for g in range(N)]
+        per_row = n_cum // fps
+        out = np.empty(c.fetch_bytes() // 8, dtype=np.complex64)
+
+        def integration(seq0):
+            seq = seq0
+            for _ in range(rows):
+                for _ in range(per_row):
+                    sl = seq % slots
+                    if seq >= slots:
+                        c.row_slot_free(seq)          # release the slot's previous use
+                    for g in range(N):
+                        masks[g][sl][:] = 0xFF         # all frames present
+                    c.row_accumulate(seq, fps)
+                    seq += 1
+                c.row_end(n_cum)
+            c.rows_wait()
+            got, _ = c.rows_fetch(out)
+            assert got == rows
+            return seq
+
+        seq = integration(0)                          # warm-up
+        t0 = time.perf_counter()
+        seq = integration(seq)
+        dt = time.perf_counter() - t0
+        frames = rows * n_cum
+        res = {"workload": f"drop-in ABI, FP32 form, {N} in, nf {nf}, {fps} frames per slot, {slots} slots, "
+                           f"{rows} rows of {n_cum} frames, pinned staging -> visibilities in host memory",
+               "ms_per_row": round(dt / rows * 1e3, 3),
+               "Msamples_per_s_per_input": round(2 * nf * frames / dt / 1e6, 1),
+               "h2d_GB_per_s": round(N * frames * L / dt / 1e9, 2),
+               "realtime_factor": round(2 * nf * frames / dt / 800e6, 2)}
+        if fp64:
+            V = out.reshape(rows, B, nf)
+            ref = fp64_visibilities(pool_host)
+            res["rel_rms_vs_fp64"] = max(rel_rms(V[r], ref) for r in range(rows))
+        return res
 
 
 def measured_traffic(N, nf, chunk, form):
@@ -455,6 +514,7 @@ def main():
     kernels = h["kernels"]
 
     batch = None if args.no_batch else batch_c5(corr, hip, args, not args.no_fp64)
+    abi = None if args.no_abi else abi_end_to_end(corr, args, pool_host, not args.no_fp64)
 
     cpu = None
     if args.cpu_seconds > 0:
@@ -483,6 +543,7 @@ def main():
         "realtime_Msps_per_input": 800.0,
         "cpu_baseline": cpu,
         "batch": batch,
+        "abi_end_to_end": abi,
     }
     print(json.dumps(out))
 

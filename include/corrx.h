@@ -138,8 +138,11 @@ corr_status corrx_gains(void *engine, const float **d_gains);
    inputs) and the channel tile of the spectrum buffer's order (that width, or
    n_freq for the plain order). The engine picks both at creation by timing
    F- and X-engine with every combination that fits its shape (cached per
-   process, form, input count, n_freq and device); set overrides the choice
-   (before the next corrx_fengine call), get reports it. */
+   process, form, input count, n_freq, launch size and device); set overrides
+   the choice (before the next corrx_fengine call), get reports it. The
+   environment variable CORRX_XTILE=<channels>,<spectra_tile> (spectra_tile 0 =
+   n_freq) pins the choice at creation, without timing, for every engine whose
+   shape it fits. */
 corr_status corrx_set_xtile(void *engine, int32_t channels, int32_t spectra_tile);
 corr_status corrx_get_xtile(void *engine, int32_t *channels, int32_t *spectra_tile);
 

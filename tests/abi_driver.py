@@ -13,8 +13,12 @@ def frame_is_missing(row, frame_in_row):
     return ((row * 3 + frame_in_row) % MISSING_EVERY) == MISSING_PHASE
 
 
-def drive(corr, cfg, x, rows, n_cum, mode=ALL_VALID, lib=None):
-    """x: int8 (rows, N, n_cum, L). Returns complex64 (rows, B, nf) from one fetch."""
+def drive(corr, cfg, x, rows, n_cum, mode=ALL_VALID, lib=None, missing=None, stats=None):
+    """x: int8 (rows, N, n_cum, L). Returns complex64 (rows, B, nf) from one fetch.
+    `missing(row, frame_in_row)` overrides the missing-frame pattern of the
+    HOST_ZEROED / MASK_MARKED modes; `stats`, a list, receives the context's
+    device statistics before it is destroyed."""
+    is_missing = missing if missing is not None else frame_is_missing
     fps = cfg.frames_per_slot
     N = cfg.n_station
     with corr.Correlator(cfg, lib=lib) as c:
@@ -29,12 +33,12 @@ def drive(corr, cfg, x, rows, n_cum, mode=ALL_VALID, lib=None):
                     frames = c.slot_frames(g, slot)
                     mask = c.slot_mask(g, slot)
                     for f in range(ft):
-                        missing = frame_is_missing(r, f0 + f)
-                        if mode != ALL_VALID and missing:
+                        gone = is_missing(r, f0 + f)
+                        if mode != ALL_VALID and gone:
                             frames[f, :] = 0 if mode == HOST_ZEROED else 0x5A
                         else:
                             frames[f, :] = x[r, g, f0 + f]
-                        if not (mode == MASK_MARKED and missing):
+                        if not (mode == MASK_MARKED and gone):
                             mask[f >> 3] |= np.uint8(1 << (f & 7))
                 c.row_accumulate(submits, ft)
                 submits += 1
@@ -46,14 +50,17 @@ def drive(corr, cfg, x, rows, n_cum, mode=ALL_VALID, lib=None):
         assert c.rows_pending() == rows
         got, out = c.rows_fetch()
         assert got == rows
+        if stats is not None:
+            stats.append(c.device_stats(0))
         return out.reshape(cfg.rows_per_fetch, c.n_baseline, cfg.n_freq)[:rows].copy()
 
 
-def valid_matrix(rows, N, n_cum):
+def valid_matrix(rows, N, n_cum, missing=None):
     """Per (row, station, frame) validity of the MASK_MARKED/HOST_ZEROED runs."""
+    is_missing = missing if missing is not None else frame_is_missing
     v = np.ones((rows, N, n_cum), dtype=np.uint8)
     for r in range(rows):
         for f in range(n_cum):
-            if frame_is_missing(r, f):
+            if is_missing(r, f):
                 v[r, :, f] = 0
     return v

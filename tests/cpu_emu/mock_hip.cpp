@@ -100,4 +100,18 @@ hipError_t hipDriverGetVersion(int *v) {
     *v = 0;
     return hipSuccess;
 }
+// Nothing is deferred here, so no graph can be captured: capture is refused and
+// the other graph calls see no graph.
+hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) { return hipErrorNotSupported; }
+hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t *g) {
+    if (g) *g = nullptr;
+    return hipErrorNotSupported;
+}
+hipError_t hipGraphInstantiate(hipGraphExec_t *e, hipGraph_t, hipGraphNode_t *, char *, size_t) {
+    if (e) *e = nullptr;
+    return hipErrorNotSupported;
+}
+hipError_t hipGraphLaunch(hipGraphExec_t, hipStream_t) { return hipErrorNotSupported; }
+hipError_t hipGraphExecDestroy(hipGraphExec_t e) { return e ? hipErrorInvalidValue : hipSuccess; }
+hipError_t hipGraphDestroy(hipGraph_t g) { return g ? hipErrorInvalidValue : hipSuccess; }
 }

@@ -19,10 +19,21 @@ enum {
     hipErrorInvalidValue = 1,
     hipErrorNotReady = 600,
     hipErrorIllegalAddress = 700,
+    hipErrorNotSupported = 801,
 };
 typedef struct MockStream *hipStream_t;
 typedef struct MockEvent *hipEvent_t;
 typedef void (*hipHostFn_t)(void *);
+// Graphs: stream operations run at once here, so there is nothing to capture;
+// hipStreamBeginCapture reports hipErrorNotSupported.
+typedef struct MockGraph *hipGraph_t;
+typedef struct MockGraphExec *hipGraphExec_t;
+typedef struct MockGraphNode *hipGraphNode_t;
+enum hipStreamCaptureMode {
+    hipStreamCaptureModeGlobal = 0,
+    hipStreamCaptureModeThreadLocal,
+    hipStreamCaptureModeRelaxed,
+};
 enum hipMemcpyKind {
     hipMemcpyHostToHost = 0,
     hipMemcpyHostToDevice = 1,
@@ -65,4 +76,10 @@ const char *hipGetErrorName(hipError_t e);
 const char *hipGetErrorString(hipError_t e);
 hipError_t hipRuntimeGetVersion(int *v);
 hipError_t hipDriverGetVersion(int *v);
+hipError_t hipStreamBeginCapture(hipStream_t s, hipStreamCaptureMode mode);
+hipError_t hipStreamEndCapture(hipStream_t s, hipGraph_t *g);
+hipError_t hipGraphInstantiate(hipGraphExec_t *e, hipGraph_t g, hipGraphNode_t *err, char *log, size_t n);
+hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t s);
+hipError_t hipGraphExecDestroy(hipGraphExec_t e);
+hipError_t hipGraphDestroy(hipGraph_t g);
 }

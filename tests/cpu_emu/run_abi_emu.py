@@ -6,6 +6,7 @@ import ctypes
 import importlib
 import os
 import sys
+import time
 
 import numpy as np
 
@@ -45,6 +46,27 @@ def ptr(a):
     return a.ctypes.data
 
 
+def skip(L):
+    """Chunks in which no station has a present frame are skipped (no copy): the
+    same bytes as transforming them as zeros, and exactly their bytes fewer copied."""
+    N, nf, fps, chunk, n_cum, rows = 3, 2048, 32, 8, 96, 2
+
+    def pattern(r, f):  # row 0: a whole slot; row 1: one whole chunk and part of another
+        return 32 <= f < 64 if r == 0 else (8 <= f < 16 or 70 <= f < 76)
+
+    rng = np.random.default_rng(5)
+    x = rng.integers(-4, 5, size=(rows, N, n_cum, 2 * nf)).astype(np.int8)
+    cfg = corr.make_config(N, nf, fps, chunk, block_slots=3, rows_per_fetch=rows)
+    zs, ms = [], []
+    z = drive(corr, cfg, x, rows, n_cum, HOST_ZEROED, lib=L, missing=pattern, stats=zs)
+    m = drive(corr, cfg, x, rows, n_cum, MASK_MARKED, lib=L, missing=pattern, stats=ms)
+    same = z.tobytes() == m.tobytes()
+    saved = zs[0].bytes_h2d - ms[0].bytes_h2d
+    want = 5 * N * (chunk * 2 * nf + (fps + 7) // 8)
+    print(f"skip: zeroed == masked {same}, copied bytes saved {saved} (want {want})", flush=True)
+    return same and saved == want
+
+
 def forms(L, form, N, nf, T, chunk, cal_frames):
     """Engine path of a precision form in the CPU build: calibrate, then
     F-engine + X-engine per chunk with a missing frame, then the dump; against
@@ -56,6 +78,7 @@ def forms(L, form, N, nf, T, chunk, cal_frames):
     valid = np.ones((N, T), dtype=bool)
     valid[1, 3] = False
     e = corr.Engine(N, nf, chunk, form, lib=L)
+    pin_ok = os.environ.get("CORRX_XTILE") != "16,16" or e.xtile() == (16, 16)
     pitch = T * 2 * nf
     e.calibrate(ptr(x), pitch, cal_frames, ptr(mask), mask_pitch, 0)
     gains = np.ctypeslib.as_array(ctypes.cast(e.gains(), ctypes.POINTER(ctypes.c_float)),
@@ -98,7 +121,7 @@ def forms(L, form, N, nf, T, chunk, cal_frames):
     missing_zero = not np.any(Yq[1, 3])
     e64 = rel_rms(V, ref)
     bound = 1e-3 if form == corr.FORM_FP16 else 0.3
-    ok = ok and missing_zero and e64 < bound and np.all(gains > 0) and same
+    ok = ok and missing_zero and e64 < bound and np.all(gains > 0) and same and pin_ok
     print(f"form {form} N={N} nf={nf} T={T} chunk={chunk}: vs oracle on engine spectra {eq:.3e}, "
           f"vs FP64 {e64:.3e}, missing frame zero {missing_zero}, integrate == loop {same}", flush=True)
     return ok
@@ -127,13 +150,67 @@ def integrate_fp32(L, N, nf, T, chunk):
     return same and err < 1e-5
 
 
+def batch(L):
+    """corrx_batch_run in direct mode (two engines of different shapes and forms)
+    against each engine alone: the same bytes. Graph mode needs a runtime that
+    defers work, which the stand-in runtime does not: it must fail cleanly."""
+    cases = [(corr.FORM_FP16, 3, 512, 8, 20), (corr.FORM_FP32, 2, 512, 16, 33)]
+    subs = []
+    for i, (form, N, nf, chunk, T) in enumerate(cases):
+        x = oracle.gen_int8(200 + i, N, T, 2 * nf, 0.3, max_delay=4)
+        e = corr.Engine(N, nf, chunk, form, lib=L)
+        if form != corr.FORM_FP32:
+            e.calibrate(ptr(x), T * 2 * nf, T)
+        B = N * (N + 1) // 2
+        nb = B * nf * e.integrator_bytes()
+        subs.append(dict(x=x, e=e, T=T, nf=nf, B=B, pitch=T * 2 * nf,
+                         acc=np.zeros(nb // 4, dtype=np.uint32), out=np.zeros(B * nf, dtype=np.complex64)))
+    ref = []
+    for s in subs:
+        s["acc"][:] = 0
+        s["e"].integrate(ptr(s["x"]), s["pitch"], s["T"], ptr(s["acc"]))
+        s["e"].dump(ptr(s["acc"]), s["T"], ptr(s["out"]))
+        ref.append((s["acc"].copy(), s["out"].copy()))
+    b = corr.Batch([s["e"] for s in subs], lib=L)
+    items = [corr.Batch.item(ptr(s["x"]), s["pitch"], s["T"], ptr(s["acc"]), d_out=ptr(s["out"]),
+                             dump_frames=s["T"]) for s in subs]
+    for s in subs:
+        s["acc"][:] = 0xFFFFFFFF    # the run must zero the integrator itself
+        s["out"][:] = 0
+    b.run(items, graph=False)
+    same = all(np.array_equal(s["acc"], r[0]) and np.array_equal(s["out"], r[1]) for s, r in zip(subs, ref))
+    graph_refused = False
+    try:
+        b.run(items, graph=True)
+    except corr.CorrError:
+        graph_refused = True
+    b.close()
+    for s in subs:
+        s["e"].close()
+    print(f"batch direct == engines alone {same}; graph refused by the stand-in runtime {graph_refused}",
+          flush=True)
+    return same and graph_refused
+
+
+def timed(fn, *args):
+    t0 = time.perf_counter()
+    r = fn(*args)
+    print(f"  ({fn.__name__}: {time.perf_counter() - t0:.1f} s)", flush=True)
+    return r
+
+
 def main():
+    # The stand-in runtime's timer reads zero, so tuning the X-engine tile would
+    # only cost emulation time here: the tile is pinned (the GPU tests tune).
+    os.environ["CORRX_XTILE"] = "16,16"
     L = corr.declare(ctypes.CDLL(os.path.join(HERE, "_build", "libcorr_cuda_emu.so")))
-    ok = golden(L, 4, 2048, 90, 32, 2)
-    ok = golden(L, 5, 4096, 37, 16, 3) and ok
-    ok = forms(L, corr.FORM_FP16, 5, 512, 70, 32, 20) and ok
-    ok = forms(L, corr.FORM_INT8, 6, 512, 70, 32, 70) and ok
-    ok = integrate_fp32(L, 3, 512, 45, 8) and ok
+    ok = timed(golden, L, 4, 2048, 90, 32, 2)
+    ok = timed(golden, L, 5, 4096, 37, 16, 3) and ok
+    ok = timed(skip, L) and ok
+    ok = timed(forms, L, corr.FORM_FP16, 5, 512, 70, 32, 20) and ok
+    ok = timed(forms, L, corr.FORM_INT8, 6, 512, 70, 32, 70) and ok
+    ok = timed(integrate_fp32, L, 3, 512, 45, 8) and ok
+    ok = timed(batch, L) and ok
     print("OK" if ok else "FAILED")
     return 0 if ok else 1
 

@@ -87,6 +87,7 @@ static int check_int8(int nst, int nf, int frames, int cap, int ch = 16, bool pl
                 if (g.x != sr || g.y != si) ++bad;
             }
     std::printf("int8 N=%d nf=%d frames=%d ch=%d tile=%d: %d mismatches\n", nst, nf, frames, ch, tl, bad);
+    std::fflush(stdout);
     return bad;
 }
 
@@ -125,31 +126,32 @@ static int check_fp16(int nst, int nf, int frames, int cap, int ch = 16, bool pl
             }
     std::printf("fp16 N=%d nf=%d frames=%d ch=%d tile=%d: worst %.3e, %d bad\n", nst, nf, frames, ch, tl, worst,
                 bad);
+    std::fflush(stdout);
     return bad;
 }
 
 int main() {
     int bad = 0;
-    // nf 256: 16 channel tiles (XCD mapping on); nf 48: 3 tiles (identity)
+    // nf 256: 16 channel tiles (XCD mapping permutes); nf 128: 8, nf 48: 3 (identity)
     bad += check_int8(1, 48, 3, 4);
     bad += check_int8(4, 256, 70, 72);
     bad += check_int8(9, 48, 130, 130);
-    bad += check_int8(24, 256, 64, 64);
+    bad += check_int8(24, 128, 64, 64);
     bad += check_int8(30, 48, 20, 20);
     bad += check_int8(48, 48, 65, 66);
     bad += check_fp16(1, 48, 1, 2);
     bad += check_fp16(5, 256, 33, 40);
     bad += check_fp16(8, 48, 64, 64);
-    bad += check_fp16(24, 256, 40, 40);
+    bad += check_fp16(24, 128, 20, 20);
     bad += check_fp16(40, 48, 31, 31);
     // 32 channels per workgroup (up to 24 inputs): nf 512 = 16 tiles, nf 96 = 3
     bad += check_int8(3, 96, 70, 70, 32);
-    bad += check_int8(24, 512, 40, 40, 32);
+    bad += check_int8(24, 96, 40, 40, 32);
     bad += check_fp16(13, 96, 33, 33, 32);
-    bad += check_fp16(24, 512, 40, 40, 32);
+    bad += check_fp16(8, 512, 20, 20, 32);
     // the plain [input][frame][channel] order (a spectrum tile of n_freq)
     bad += check_int8(9, 96, 70, 72, 16, true);
-    bad += check_int8(24, 512, 40, 40, 32, true);
+    bad += check_int8(24, 96, 40, 40, 32, true);
     bad += check_fp16(5, 512, 33, 40, 16, true);
     bad += check_fp16(24, 96, 40, 40, 32, true);
     std::printf(bad ? "FAILED\n" : "OK\n");

@@ -278,3 +278,49 @@ def test_c3_shape_fp16_error(corr, oracle, hip):
     print(f"  relative RMS vs FP64: FP16 form {e16:.3e}, FP32 form {e32:.3e}")
     assert e16 < 1e-3
     assert e32 < 1e-5
+
+
+@pytest.mark.parametrize("form,nf", [(1, 2048), (1, 16384), (2, 2048), (2, 16384)])
+def test_fengine_form_rounding(corr, oracle, hip, form, nf):
+    """The form's F-engine output is the FP32 form's spectrum y requantized on
+    the host: FP16 = y*g rounded to nearest even (numpy float16), int8 =
+    clamp(rint(y*g), +-7), for at least 99.9% of values and every value within
+    one FP16 ulp / one integer step (the two kernels share the FFT, so the rare
+    difference can only come from a 1-ulp FP32 difference at a rounding tie).
+    Pins the rounding mode of the device's FP16 conversion."""
+    N, T = 4, 8
+    print(f"F-engine form {form} rounding N={N} nf={nf}", flush=True)
+    x = oracle.gen_int8(21 + form, N, T, 2 * nf, 0.3, sigma=16.0, max_delay=4)
+    d_in = hip.Buffer.from_array(x)
+    pitch = T * 2 * nf
+    e32 = corr.Engine(N, nf, T, corr.FORM_FP32)
+    e32.fengine(d_in.ptr, pitch, T)
+    hip.synchronize()
+    ptr, stride = e32.spectra()
+    y = np.empty((N, stride), dtype=np.complex64)
+    hip.check(hip.rt().hipMemcpy(y.ctypes.data, ptr, y.nbytes, hip.D2H), "hipMemcpy")
+    y = y.reshape(N, T, nf)
+    e32.close()
+    eng = corr.Engine(N, nf, T, form)
+    eng.calibrate(d_in.ptr, pitch, T)
+    eng.fengine(d_in.ptr, pitch, T)
+    hip.synchronize()
+    g = np.empty((N, nf), dtype=np.float32)
+    hip.check(hip.rt().hipMemcpy(g.ctypes.data, eng.gains(), g.nbytes, hip.D2H), "hipMemcpy")
+    got = _download_spectra(hip, eng, N, T, nf, form)
+    eng.close()
+    ys = y * g[:, None, :]                       # exact: FP16 gains are powers of two
+    comp = np.stack([ys.real, ys.imag], -1).astype(np.float32)
+    have = np.stack([got.real, got.imag], -1)
+    if form == corr.FORM_FP16:
+        want = comp.astype(np.float16)
+        hb = have.astype(np.float16).view(np.int16).astype(np.int32)
+        wb = want.view(np.int16).astype(np.int32)
+        same = np.mean(hb == wb)
+        worst = int(np.max(np.abs(hb - wb)))
+    else:
+        want = np.clip(np.rint(comp), -7, 7)
+        same = np.mean(have == want)
+        worst = float(np.max(np.abs(have - want)))
+    print(f"  equal {same:.6f}, largest difference {worst}", flush=True)
+    assert same >= 0.999 and worst <= 1

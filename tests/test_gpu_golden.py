@@ -7,7 +7,9 @@ Checks, as the reference does:
   2. autocorrelation |Im| <= 1e-3 max|Re| (golden.c:336-349, 398-403);
   3. host-zeroed and mask-marked missing frames give byte-identical rows, which
      differ from the all-valid run (golden.c:351-396).
-Also checked against numpy's FP64 rfft (an independent library).
+Also checked against numpy's FP64 rfft (an independent library), and, for
+chunks in which every frame is missing, that they are skipped (no copy) with
+the same bytes as a run that transforms them as zeros.
 """
 
 import numpy as np
@@ -66,3 +68,36 @@ def test_golden(corr, oracle, N, nf, n_cum, fps, rows):
     valid = valid_matrix(rows, N, n_cum)
     ref_m = np.stack([oracle.fx_f64(x[r], frames_div=n_cum, valid=valid[r]) for r in range(rows)])
     assert rel_rms(masked, ref_m) < TOL
+
+
+def _skip_pattern(r, f):
+    """Row 0: the whole second slot (frames 32..63) missing on every station.
+    Row 1: frames 8..15 (one whole chunk) and 70..75 (part of a chunk)."""
+    if r == 0:
+        return 32 <= f < 64
+    return 8 <= f < 16 or 70 <= f < 76
+
+
+def test_whole_missing_chunks_skipped(corr, oracle):
+    """A chunk in which no station has a present frame is neither copied nor
+    transformed (the reference's next step for whole-missing slots,
+    docs/DESIGN-zero-fill-decoupling.md:219-231). The rows must equal the
+    host-zeroed run byte for byte and the FP64 oracle on the validity pattern,
+    and the copied bytes must drop by exactly the five skipped chunks."""
+    N, nf, fps, chunk, n_cum, rows = 4, 2048, 32, 8, 96, 2
+    L = 2 * nf
+    print(f"skip whole-missing chunks N={N} nf={nf} fps={fps} chunk={chunk}", flush=True)
+    x = golden_input(rows, N, n_cum, L, seed=77)
+    cfg = corr.make_config(N, nf, fps, chunk, block_slots=3, rows_per_fetch=rows)
+    zs, ms = [], []
+    zeroed = drive(corr, cfg, x, rows, n_cum, HOST_ZEROED, missing=_skip_pattern, stats=zs)
+    masked = drive(corr, cfg, x, rows, n_cum, MASK_MARKED, missing=_skip_pattern, stats=ms)
+    assert zeroed.tobytes() == masked.tobytes(), "skipping differs from transforming zeros"
+    valid = valid_matrix(rows, N, n_cum, _skip_pattern)
+    ref = np.stack([oracle.fx_f64(x[r], frames_div=n_cum, valid=valid[r]) for r in range(rows)])
+    assert rel_rms(masked, ref) < TOL
+    mask_per_slot = (fps + 7) // 8
+    per_chunk = N * (chunk * L + mask_per_slot)
+    print(f"  copied bytes: host-zeroed {zs[0].bytes_h2d}, mask-marked {ms[0].bytes_h2d}", flush=True)
+    assert zs[0].bytes_h2d - ms[0].bytes_h2d == 5 * per_chunk
+    assert zs[0].transfers - ms[0].transfers == 5
