@@ -283,23 +283,31 @@ CORR_HD inline void xengine_mfma_block(int tid, int bx, int n_tiles, uint8_t *ld
 
     // Items of this lane: tid, tid + kThreads (the second only for > 32 inputs).
     // The next step's loads are issued right after this step's stores, into
-    // the same registers, so they are in flight during the MFMA phase.
-    // With two items per lane (more than 32 inputs) the registers of a second
-    // staged step do not fit beside the accumulators: those load each step in turn.
+    // the same registers, so they are in flight during the MFMA phase. Two
+    // staged items (64 registers) fit beside the accumulators up to five row
+    // tiles (40 inputs, 120 accumulator registers); at six row tiles they do
+    // not, and the items load each step in turn.
     constexpr int IPT = (CH * 8 * NRT + kThreads - 1) / kThreads;
-    constexpr bool kAhead = IPT == 1;
+    constexpr bool kAhead = IPT == 1 || (IPT == 2 && NRT <= 5);
     Stage stg[IPT];
     if constexpr (kAhead) {
-        if (tid < n_items) St::load(tid, spectra, nf, fcap, spec_tile, frames, 0, c0, stg[0]);
+#pragma unroll
+        for (int u = 0; u < IPT; ++u)
+            if (tid + u * kThreads < n_items)
+                St::load(tid + u * kThreads, spectra, nf, fcap, spec_tile, frames, 0, c0, stg[u]);
     }
 
     for (int f0 = 0; f0 < frames; f0 += T::kFramesPerStep) {
         sync();
         if constexpr (kAhead) {
-            if (tid < n_items) {
-                St::store(tid, lds, cpitch, stg[0]);
-                if (f0 + T::kFramesPerStep < frames)
-                    St::load(tid, spectra, nf, fcap, spec_tile, frames, f0 + T::kFramesPerStep, c0, stg[0]);
+#pragma unroll
+            for (int u = 0; u < IPT; ++u) {
+                const int item = tid + u * kThreads;
+                if (item < n_items) {
+                    St::store(item, lds, cpitch, stg[u]);
+                    if (f0 + T::kFramesPerStep < frames)
+                        St::load(item, spectra, nf, fcap, spec_tile, frames, f0 + T::kFramesPerStep, c0, stg[u]);
+                }
             }
         } else {
 #pragma unroll 1

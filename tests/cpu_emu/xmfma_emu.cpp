@@ -138,12 +138,13 @@ int main() {
     bad += check_int8(9, 48, 130, 130);
     bad += check_int8(24, 128, 64, 64);
     bad += check_int8(30, 48, 20, 20);
+    bad += check_int8(36, 48, 70, 70);  // five row tiles, two items per lane, prefetched
     bad += check_int8(48, 48, 65, 66);
     bad += check_fp16(1, 48, 1, 2);
     bad += check_fp16(5, 256, 33, 40);
     bad += check_fp16(8, 48, 64, 64);
     bad += check_fp16(24, 128, 20, 20);
-    bad += check_fp16(40, 48, 31, 31);
+    bad += check_fp16(40, 48, 70, 70);  // five row tiles, two items per lane, prefetched
     // 32 channels per workgroup (up to 24 inputs): nf 512 = 16 tiles, nf 96 = 3
     bad += check_int8(3, 96, 70, 70, 32);
     bad += check_int8(24, 96, 40, 40, 32);
