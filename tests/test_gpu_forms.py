@@ -324,3 +324,20 @@ def test_fengine_form_rounding(corr, oracle, hip, form, nf):
         worst = float(np.max(np.abs(have - want)))
     print(f"  equal {same:.6f}, largest difference {worst}", flush=True)
     assert same >= 0.999 and worst <= 1
+
+
+def test_xtile_pinned_by_environment(corr, hip, monkeypatch):
+    """CORRX_XTILE=<channels>,<spectra tile> (0 = n_freq) pins the tile choice
+    at creation where it fits the shape; where it does not, the tuner runs."""
+    monkeypatch.setenv("CORRX_XTILE", "16,0")
+    eng = corr.Engine(4, 512, 8, corr.FORM_FP16)
+    assert eng.xtile() == (16, 512)
+    eng.close()
+    monkeypatch.setenv("CORRX_XTILE", "32,32")
+    eng = corr.Engine(4, 1024, 8, corr.FORM_INT8)
+    assert eng.xtile() == (32, 32)
+    eng.close()
+    eng = corr.Engine(40, 512, 8, corr.FORM_FP16)     # 32 channels do not fit 40 inputs
+    ch, tl = eng.xtile()
+    assert ch == 16 and tl in (16, 512)
+    eng.close()
