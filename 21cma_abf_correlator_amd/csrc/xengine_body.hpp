@@ -4,7 +4,8 @@
 // acc[b][c] += sum_t Y_j[t][c] conj(Y_k[t][c]) for baseline b = (j,k), j <= k,
 // with Re = xr*yr + xi*yi and Im = xi*yr - xr*yi (reference kernels.cu:243-246),
 // summed in frame order over the call's frames and added to acc once per call
-// (reference kernels.cu:216-264).
+// (reference kernels.cu:216-264). Each product is fused into the running sum
+// (two FMAs per component), a rounding order of its own within FP32 accuracy.
 //
 // Workgroup: 64 channels (one per lane) x W waves. Stations are taken in blocks
 // of 4; a wave owns NPAIR (jb <= kb) block pairs and a lane keeps 16 complex
@@ -104,8 +105,11 @@ CORR_HD inline void xengine_fp32_block(int tid, int nthreads, int bx, int by, fl
                 for (int e = 0; e < 4; ++e)
 #pragma unroll
                     for (int g = 0; g < 4; ++g) {
-                        a[q][e * 4 + g].x += xj[e].x * xk[g].x + xj[e].y * xk[g].y;
-                        a[q][e * 4 + g].y += xj[e].y * xk[g].x - xj[e].x * xk[g].y;
+                        // each product fused into the running sum: two FMAs per
+                        // component instead of a multiply, an FMA and an add
+                        float2 &s = a[q][e * 4 + g];
+                        s.x = __builtin_fmaf(xj[e].x, xk[g].x, __builtin_fmaf(xj[e].y, xk[g].y, s.x));
+                        s.y = __builtin_fmaf(xj[e].y, xk[g].x, __builtin_fmaf(-xj[e].x, xk[g].y, s.y));
                     }
             }
         }
