@@ -30,6 +30,8 @@ corr_status Batch::create(Engine *const *engines, int n, Batch **out) {
     if (st == CORR_OK)
         st = hip_check(hipEventCreateWithFlags(&b->fork_, hipEventDisableTiming), "hipEventCreate");
     if (st == CORR_OK)
+        st = hip_check(hipEventCreateWithFlags(&b->done_, hipEventDisableTiming), "hipEventCreate");
+    if (st == CORR_OK)
         st = hip_check(hipStreamCreateWithFlags(&b->cap_, hipStreamNonBlocking), "hipStreamCreate(batch capture)");
     if (st != CORR_OK) {
         delete b;
@@ -49,10 +51,15 @@ Batch::~Batch() {
     for (hipStream_t s : branch_)
         if (s != nullptr) hipStreamDestroy(s);
     if (fork_ != nullptr) hipEventDestroy(fork_);
+    if (done_ != nullptr) hipEventDestroy(done_);
     if (cap_ != nullptr) hipStreamDestroy(cap_);
 }
 
+// The last launch of the graph may still run on the caller's stream: it ends
+// (done_, recorded behind it) before the graph is destroyed.
 void Batch::drop_graph() {
+    if (exec_ != nullptr && launched_) hipEventSynchronize(done_);
+    launched_ = false;
     if (exec_ != nullptr) hipGraphExecDestroy(exec_);
     if (graph_ != nullptr) hipGraphDestroy(graph_);
     exec_ = nullptr;
@@ -140,7 +147,9 @@ corr_status Batch::run(const corrx_batch_item *items, bool graph, hipStream_t s)
     const bool same = exec_ != nullptr && captured_.size() == eng_.size() &&
                       std::memcmp(captured_.data(), items, sizeof(corrx_batch_item) * eng_.size()) == 0;
     if (!same && (st = capture(items)) != CORR_OK) return st;
-    return hip_check(hipGraphLaunch(exec_, s), "hipGraphLaunch");
+    if ((st = hip_check(hipGraphLaunch(exec_, s), "hipGraphLaunch")) != CORR_OK) return st;
+    launched_ = true;
+    return hip_check(hipEventRecord(done_, s), "hipEventRecord(graph launch)");
 }
 
 }  // namespace corr

@@ -51,6 +51,8 @@ class Batch {
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;
     std::vector<corrx_batch_item> captured_;  ///< the arguments the graph was made with
+    bool launched_ = false;                   ///< the graph has been launched
+    hipEvent_t done_ = nullptr;               ///< recorded behind its last launch
 };
 
 }  // namespace corr
