@@ -103,7 +103,7 @@ ABI_SYMBOLS = (
 ENGINE_SYMBOLS = (
     "corrx_engine_create", "corrx_engine_destroy", "corrx_fengine", "corrx_xengine",
     "corrx_integrate", "corrx_batch_create", "corrx_batch_destroy", "corrx_batch_run", "corrx_scale", "corrx_spectra", "corrx_integrator_bytes", "corrx_dump", "corrx_calibrate",
-    "corrx_set_gains", "corrx_gains", "corrx_set_xtile", "corrx_get_xtile",
+    "corrx_set_gains", "corrx_gains", "corrx_set_xtile", "corrx_get_xtile", "corrx_create_packed",
 )
 
 _lib = None
@@ -181,6 +181,7 @@ def declare(L):
         "corrx_xengine": ([vp, vp, i32, vp], st),
         "corrx_integrate": ([vp, vp, sz, vp, sz, i32, ctypes.c_int64, vp, vp], st),
         "corrx_batch_create": ([ctypes.POINTER(vp), i32, ctypes.POINTER(vp)], st),
+        "corrx_create_packed": ([ctypes.POINTER(corr_config), i32, ctypes.POINTER(vp)], st),
         "corrx_batch_destroy": ([vp], st),
         "corrx_batch_run": ([vp, ctypes.POINTER(corrx_batch_item), i32, vp], st),
         "corrx_scale": ([vp, sz, u64, vp], st),
@@ -254,22 +255,40 @@ def make_config(n_station, n_freq, frames_per_slot, chunk_frames, block_slots=3,
     return c
 
 
-def create_raw(cfg, lib=None):
-    """corr_create; returns (status, handle or None) without raising."""
+def create_raw(cfg, lib=None, sample_bits=8):
+    """corr_create (sample_bits 8) or corrx_create_packed; returns (status,
+    handle or None) without raising."""
     h = ctypes.c_void_p()
-    s = (lib or load()).corr_create(ctypes.byref(cfg), ctypes.byref(h))
+    L = lib or load()
+    if sample_bits == 8:
+        s = L.corr_create(ctypes.byref(cfg), ctypes.byref(h))
+    else:
+        s = L.corrx_create_packed(ctypes.byref(cfg), sample_bits, ctypes.byref(h))
     return s, (h.value if h.value else None)
+
+
+def pack4(x):
+    """int8 samples in [-8, 7], last axis even -> the 4-bit staging bytes of
+    corrx_create_packed: sample 2m in bits 0-3, sample 2m+1 in bits 4-7 of byte m."""
+    x = np.asarray(x)
+    if x.min(initial=0) < -8 or x.max(initial=0) > 7:
+        raise ValueError("4-bit samples must lie in [-8, 7]")
+    u = (x.astype(np.int16) & 0xF).astype(np.uint8)
+    return (u[..., 0::2] | (u[..., 1::2] << 4)).astype(np.uint8).view(np.int8)
 
 
 class Correlator:
     """One correlator context (reference corr-rs/src/gpu.rs:32-333). `lib` is the
-    loaded library to drive (default: this package's libcorr_cuda.so)."""
+    loaded library to drive (default: this package's libcorr_cuda.so);
+    `sample_bits` 4 makes a context with 4-bit packed staging (frames of
+    n_freq bytes, see pack4)."""
 
-    def __init__(self, cfg, lib=None):
+    def __init__(self, cfg, lib=None, sample_bits=8):
         self.cfg = cfg
+        self.sample_bits = sample_bits
         self._lib = lib or load()
-        s, h = create_raw(cfg, self._lib)
-        check(s, "corr_create", self._lib)
+        s, h = create_raw(cfg, self._lib, sample_bits)
+        check(s, "corr_create" if sample_bits == 8 else "corrx_create_packed", self._lib)
         self.h = h
         self._staging = [self.staging(i) for i in range(cfg.n_station)]
 
@@ -324,7 +343,8 @@ class Correlator:
         return np.ctypeslib.as_array(s.base, shape=(s.total_bytes,))
 
     def slot_frames(self, station, slot):
-        """(frames_per_slot, 2*n_freq) int8 view of one staging slot."""
+        """(frames_per_slot, frame_stride) int8 view of one staging slot:
+        2*n_freq samples per frame, or n_freq bytes of packed 4-bit samples."""
         s = self._staging[station]
         v = self.station_view(station)
         a = v[slot * s.slot_stride: (slot + 1) * s.slot_stride]

@@ -46,13 +46,13 @@ corr_status hip_check(hipError_t e, const char *what) {
 
 bool fft_size_supported(int n_freq) {
     if (n_freq < 512 || n_freq > 16384) return false;
-    return (n_freq & (n_freq - 1)) == 0;
+    if ((n_freq & (n_freq - 1)) == 0) return true;
+    return n_freq == 6144 || n_freq == 10240 || n_freq == 12288 || n_freq == 14336;
 }
 
-static int ilog2(int v) {
-    int l = 0;
-    while ((1 << l) < v) ++l;
-    return l;
+int fft_twiddle_elems(int n_freq) {
+    const int p = n_freq & -n_freq;  // largest power of two dividing n_freq
+    return p == n_freq ? n_freq : n_freq + p;
 }
 
 corr_status Engine::create(const EngineShape &shape, Engine **out) {
@@ -62,8 +62,8 @@ corr_status Engine::create(const EngineShape &shape, Engine **out) {
     if (shape.chunk_frames < 1) return fail(CORR_ERR_ARG, "engine: chunk_frames must be >= 1");
     if (!fft_size_supported(shape.n_freq))
         return failf(CORR_ERR_NOT_IMPL,
-                     "n_freq=%d: this build transforms power-of-two n_freq in [512, 16384] "
-                     "only (FFT length 2*n_freq)",
+                     "n_freq=%d: this build transforms n_freq = 2^k in [512, 16384] and "
+                     "6144, 10240, 12288, 14336 only (FFT length 2*n_freq)",
                      shape.n_freq);
     if (shape.form != Form::kFP32 && shape.form != Form::kFP16 && shape.form != Form::kINT8)
         return failf(CORR_ERR_ARG, "engine: unknown form %d", (int)shape.form);
@@ -86,10 +86,9 @@ corr_status Engine::init() {
     const int N = shape_.n_station;
     const int nf = shape_.n_freq;
     n_baseline_ = N * (N + 1) / 2;
-    log2_nf_ = ilog2(nf);
 
     corr_status st;
-    if ((st = hip_check(hipMalloc(reinterpret_cast<void **>(&d_tw_), sizeof(float2) * nf),
+    if ((st = hip_check(hipMalloc(reinterpret_cast<void **>(&d_tw_), sizeof(float2) * (size_t)fft_twiddle_elems(nf)),
                         "hipMalloc(twiddles)")) != CORR_OK ||
         (st = hip_check(hipMalloc(reinterpret_cast<void **>(&d_wl_), sizeof(float2) * nf),
                         "hipMalloc(r2c twiddles)")) != CORR_OK)
@@ -245,7 +244,7 @@ corr_status Engine::tune_xtile() {
         (st = hip_check(launch_fill(in, in_bytes / 4, 0u, s), "fill_kernel(input)")) == CORR_OK &&
         (st = hip_check(launch_fill(integ, integ_bytes / 4, 0u, s), "fill_kernel(integrator)")) == CORR_OK) {
         auto round = [&](const std::pair<int, int> &c) {
-            corr_status r = hip_check(launch_fengine_form((int)shape_.form, log2_nf_, static_cast<const int8_t *>(in),
+            corr_status r = hip_check(launch_fengine_form((int)shape_.form, nf, static_cast<const int8_t *>(in),
                                                           0, nullptr, 0, d_tw_, d_wl_, d_gain_, d_spec_,
                                                           spectra_station_stride(), c.second, frames, N, s),
                                       "fengine_form_kernel");
@@ -372,7 +371,7 @@ corr_status Engine::integrate(const int8_t *d_in, size_t station_pitch, const ui
     if (frames < 0) return fail(CORR_ERR_ARG, "integrate: negative frame count");
     if (d_in == nullptr) return fail(CORR_ERR_ARG, "integrate: input is NULL");
     if (integ == nullptr) return fail(CORR_ERR_ARG, "integrate: integrator is NULL");
-    if (station_pitch / (2u * (size_t)shape_.n_freq) < (uint64_t)frames)
+    if (station_pitch / frame_bytes() < (uint64_t)frames)
         return fail(CORR_ERR_ARG, "integrate: station pitch shorter than the frames it holds");
     if (mask_bit0 < 0) return fail(CORR_ERR_ARG, "integrate: negative mask bit offset");
     if (d_mask != nullptr && mask_pitch * 8u < (uint64_t)mask_bit0 + (uint64_t)frames)
@@ -388,7 +387,7 @@ corr_status Engine::integrate(const int8_t *d_in, size_t station_pitch, const ui
         for (int64_t f0 = 0; f0 < frames && st == CORR_OK; f0 += chunk) {
             const int n = frames - f0 < chunk ? (int)(frames - f0) : chunk;
             const int bit0 = d_mask != nullptr ? mask_bit0 + (int)f0 : 0;
-            if ((st = fengine_to(d_spec_, d_in + (size_t)f0 * 2u * (size_t)shape_.n_freq, station_pitch, d_mask,
+            if ((st = fengine_to(d_spec_, d_in + (size_t)f0 * frame_bytes(), station_pitch, d_mask,
                                  mask_pitch, bit0, n, s)) == CORR_OK)
                 st = xengine_from(d_spec_, integ, n, s);
         }
@@ -403,7 +402,7 @@ corr_status Engine::integrate(const int8_t *d_in, size_t station_pitch, const ui
         const int n = frames - f0 < chunk ? (int)(frames - f0) : chunk;
         const int bit0 = d_mask != nullptr ? mask_bit0 + (int)f0 : 0;
         if ((k < 2 || (st = hip_check(hipStreamWaitEvent(s, ev_x_[b], 0), "hipStreamWaitEvent")) == CORR_OK) &&
-            (st = fengine_to(spec[b], d_in + (size_t)f0 * 2u * (size_t)shape_.n_freq, station_pitch, d_mask,
+            (st = fengine_to(spec[b], d_in + (size_t)f0 * frame_bytes(), station_pitch, d_mask,
                              mask_pitch, bit0, n, s)) == CORR_OK &&
             (st = hip_check(hipEventRecord(ev_f_[b], s), "hipEventRecord")) == CORR_OK &&
             (st = hip_check(hipStreamWaitEvent(xs_, ev_f_[b], 0), "hipStreamWaitEvent")) == CORR_OK &&
@@ -426,7 +425,7 @@ corr_status Engine::fengine_to(void *spec, const int8_t *d_in, size_t station_pi
         return failf(CORR_ERR_ARG, "fengine: frames=%d outside [0, %d]", frames,
                      shape_.chunk_frames);
     if (d_in == nullptr) return fail(CORR_ERR_ARG, "fengine: input is NULL");
-    if (station_pitch < (size_t)frames * 2u * (size_t)shape_.n_freq)
+    if (station_pitch < (size_t)frames * frame_bytes())
         return fail(CORR_ERR_ARG, "fengine: station pitch shorter than the frames it holds");
     if (mask_bit0 < 0) return fail(CORR_ERR_ARG, "fengine: negative mask bit offset");
     if (d_mask != nullptr && mask_pitch * 8u < (size_t)(mask_bit0 + frames))
@@ -443,11 +442,11 @@ corr_status Engine::fengine_to(void *spec, const int8_t *d_in, size_t station_pi
         valid = d_valid_;
     }
     if (shape_.form == Form::kFP32)
-        return hip_check(launch_fengine_fp32(log2_nf_, d_in, station_pitch, valid, valid_stride, d_tw_,
+        return hip_check(launch_fengine_fp32(shape_.n_freq, d_in, station_pitch, valid, valid_stride, d_tw_,
                                              d_wl_, static_cast<float2 *>(spec),
                                              spectra_station_stride(), frames, shape_.n_station, s),
                          "fengine_fp32_kernel");
-    return hip_check(launch_fengine_form((int)shape_.form, log2_nf_, d_in, station_pitch, valid,
+    return hip_check(launch_fengine_form((int)shape_.form, shape_.n_freq, d_in, station_pitch, valid,
                                          valid_stride, d_tw_, d_wl_, d_gain_, spec,
                                          spectra_station_stride(), spec_tile_, frames, shape_.n_station, s),
                      "fengine_form_kernel");
@@ -505,7 +504,7 @@ corr_status Engine::calibrate(const int8_t *d_in, size_t station_pitch, const ui
                 return st;
             valid = d_valid_;
         }
-        if ((st = hip_check(launch_fengine_fp32(log2_nf_, d_in + (size_t)b0 * 2u * (size_t)nf, station_pitch,
+        if ((st = hip_check(launch_fengine_fp32(nf, d_in + (size_t)b0 * 2u * (size_t)nf, station_pitch,
                                                 valid, valid_stride, d_tw_, d_wl_, spec, cal_stride, nb, N, s),
                             "fengine_fp32_kernel")) != CORR_OK ||
             (st = hip_check(launch_power(nf, nb, N, spec, cal_stride, valid, valid_stride, d_power_, d_count_, s),

@@ -46,8 +46,15 @@ constexpr int kMaxBaselinesAbi = 1024;
 /// Largest input count of the FP16/int8 forms (2*n_station <= 96 Gram rows).
 constexpr int kMaxStationMfma = 48;
 
-/// FFT sizes this build transforms: n_freq = 2^k with 512 <= n_freq <= 16384.
+/// FFT sizes this build transforms: n_freq = 2^k with 512 <= n_freq <= 16384,
+/// and the multiples of 2048 below 16384 with an odd factor, 6144 (3 x 2048),
+/// 10240 (5 x 2048), 12288 (3 x 4096) and 14336 (7 x 2048).
 bool fft_size_supported(int n_freq);
+
+/// Twiddle table entries of a supported n_freq: exp(-2 pi i m/n_freq) for
+/// m < n_freq, then, for n_freq = M * P with an odd M > 1, exp(-2 pi i m/P)
+/// for m < P (the power-of-two sub-transforms).
+int fft_twiddle_elems(int n_freq);
 
 struct EngineShape {
     int n_station = 0;
@@ -64,6 +71,8 @@ class Engine {
     ~Engine();
 
     const EngineShape &shape() const { return shape_; }
+    /// Input bytes per frame: 2*n_freq int8 samples.
+    size_t frame_bytes() const { return 2u * (size_t)shape_.n_freq; }
     int n_baseline() const { return n_baseline_; }
 
     /// F-engine over `frames` frames of every station.
@@ -157,14 +166,13 @@ class Engine {
 
     EngineShape shape_{};
     int n_baseline_ = 0;
-    int log2_nf_ = 0;
 
     int cal_frames_ = 1;        ///< frames per calibration batch (FP32 spectra in d_spec_)
     int x_tile_ = 16;           ///< channels per MFMA X-engine workgroup
     int spec_tile_ = 16;        ///< channel tile of the spectrum order (x_tile_ or n_freq)
     corr_status tune_xtile();
 
-    float2 *d_tw_ = nullptr;    ///< exp(-2 pi i m / n_freq), m < n_freq
+    float2 *d_tw_ = nullptr;    ///< fft_twiddle_elems(n_freq) entries
     float2 *d_wl_ = nullptr;    ///< exp(-2 pi i k / (2 n_freq)), k < n_freq
     void *d_spec_ = nullptr;
     int32_t *d_valid_ = nullptr;  ///< expanded validity, [station][chunk_frames]
@@ -183,10 +191,15 @@ class Engine {
 };
 
 // Kernel launchers (engine_kernels.hip).
+/// 4-bit packed frames (n_freq bytes each, in_pitch bytes per station) to int8
+/// frames (2*n_freq bytes, out_pitch per station), `frames` frames of each station.
+hipError_t launch_unpack4(const int8_t *in, size_t in_pitch, int8_t *out, size_t out_pitch, int frames,
+                          int n_freq, int n_station, hipStream_t s);
+/// tw: fft_twiddle_elems(n_freq) entries; wl: n_freq.
 hipError_t launch_twiddles(float2 *tw, float2 *wl, int n_freq, hipStream_t s);
 hipError_t launch_validity(const uint8_t *mask, size_t mask_pitch, int bit0, int frames,
                            int n_station, int32_t *valid, size_t valid_stride, hipStream_t s);
-hipError_t launch_fengine_fp32(int log2_nf, const int8_t *in, size_t station_pitch,
+hipError_t launch_fengine_fp32(int n_freq, const int8_t *in, size_t station_pitch,
                                const int32_t *valid, size_t valid_stride, const float2 *tw,
                                const float2 *wl, float2 *out, size_t out_station_stride,
                                int frames, int n_station, hipStream_t s);
@@ -195,7 +208,7 @@ hipError_t launch_xengine_fp32(const float2 *y, size_t y_station_stride, int n_f
                                int npair, int waves, int groups, int frames_lds, float2 *acc,
                                hipStream_t s);
 hipError_t launch_scale(float2 *acc, size_t n, float s, hipStream_t st);
-hipError_t launch_fengine_form(int form, int log2_nf, const int8_t *in, size_t station_pitch,
+hipError_t launch_fengine_form(int form, int n_freq, const int8_t *in, size_t station_pitch,
                                const int32_t *valid, size_t valid_stride, const float2 *tw,
                                const float2 *wl, const float *gain, void *out,
                                size_t out_station_stride, int tile, int frames, int n_station,

@@ -102,6 +102,45 @@ fengine_form_kernel(const int8_t *__restrict__ in, size_t station_pitch,
                                          out, out_station_stride, tile_shift);
 }
 
+// F-engine at n_freq = M * 2^LOG2P (M = 3, 5, 7): the radix-M stage, then M
+// power-of-two sub-transforms side by side in LDS (fft_body.hpp); one
+// workgroup per (frame, station), N/16 lanes, up to 122 KB of LDS.
+template <int M, int LOG2P>
+__global__ void __launch_bounds__((M << LOG2P) / 16)
+fengine_mixed_fp32_kernel(const int8_t *__restrict__ in, size_t station_pitch,
+                          const int32_t *__restrict__ valid, size_t valid_stride,
+                          const float2 *__restrict__ tw, const float2 *__restrict__ wl,
+                          float2 *__restrict__ out, size_t out_station_stride) {
+    __shared__ float2 lds[fft::MixedSizes<M, LOG2P>::kLds];
+    DeviceSync sync;
+    fft::fengine_mixed_fp32_block<M, LOG2P>((int)threadIdx.x, (int)blockIdx.x, (int)blockIdx.y, lds, sync, in,
+                                            station_pitch, valid, valid_stride, tw, wl, out, out_station_stride);
+}
+
+template <int M, int LOG2P, int FORM>
+__global__ void __launch_bounds__((M << LOG2P) / 16)
+fengine_mixed_form_kernel(const int8_t *__restrict__ in, size_t station_pitch,
+                          const int32_t *__restrict__ valid, size_t valid_stride,
+                          const float2 *__restrict__ tw, const float2 *__restrict__ wl,
+                          const float *__restrict__ gain, void *__restrict__ out,
+                          size_t out_station_stride, int tile) {
+    __shared__ float2 lds[fft::MixedSizes<M, LOG2P>::kLds];
+    DeviceSync sync;
+    fft::fengine_mixed_form_block<M, LOG2P, FORM>((int)threadIdx.x, (int)blockIdx.x, (int)blockIdx.y, lds, sync,
+                                                  in, station_pitch, valid, valid_stride, tw, wl, gain, out,
+                                                  out_station_stride, tile);
+}
+
+// 4-bit packed samples -> int8, one 32-bit word (8 samples) per lane;
+// grid (ceil(words/256), stations).
+constexpr unsigned kUnpackBlock = 256;
+__global__ void __launch_bounds__(kUnpackBlock)
+unpack4_kernel(const uint32_t *__restrict__ in, size_t in_pitch, uint64_t *__restrict__ out, size_t out_pitch,
+               size_t words) {
+    fft::unpack4((size_t)blockIdx.x * kUnpackBlock + threadIdx.x, (int)blockIdx.y, in, in_pitch, out, out_pitch,
+                 words);
+}
+
 // Validity bits -> int32 per (station, frame); grid (ceil(frames/256), stations).
 constexpr unsigned kValidityBlock = 256;
 __global__ void __launch_bounds__(kValidityBlock)
@@ -111,11 +150,12 @@ validity_kernel(const uint8_t *__restrict__ mask, size_t mask_pitch, int bit0, i
                          mask, mask_pitch, bit0, valid, valid_stride);
 }
 
-// Twiddle tables in FP64: tw[m] = exp(-2 pi i m/n), wl[k] = exp(-2 pi i k/(2n)).
+// Twiddle tables in FP64: tw[m] = exp(-2 pi i m/n), wl[k] = exp(-2 pi i k/(2n)),
+// and for p > 0 (n = M p, odd M) tw[n + m] = exp(-2 pi i m/p), m < p < n.
 // The block size is a compile-time constant and the index is unsigned, so the
 // kernel depends on no runtime-filled argument for its bounds.
 constexpr unsigned kTwiddleBlock = 256;
-__global__ void __launch_bounds__(kTwiddleBlock) twiddle_kernel(float2 *tw, float2 *wl, unsigned n) {
+__global__ void __launch_bounds__(kTwiddleBlock) twiddle_kernel(float2 *tw, float2 *wl, unsigned n, unsigned p) {
     const unsigned i = blockIdx.x * kTwiddleBlock + threadIdx.x;
     if (i >= n) return;
     double s, c;
@@ -123,6 +163,10 @@ __global__ void __launch_bounds__(kTwiddleBlock) twiddle_kernel(float2 *tw, floa
     tw[i] = make_float2((float)c, (float)s);
     sincospi(-(double)i / (double)n, &s, &c);
     wl[i] = make_float2((float)c, (float)s);
+    if (i < p) {
+        sincospi(-2.0 * (double)i / (double)p, &s, &c);
+        tw[n + i] = make_float2((float)c, (float)s);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -240,17 +284,42 @@ hipError_t launch_fengine_form_t(const int8_t *in, size_t station_pitch, const i
     return hipGetLastError();
 }
 
+template <int M, int LOG2P, int FORM>
+hipError_t launch_fengine_mixed_form_t(const int8_t *in, size_t station_pitch, const int32_t *valid,
+                                       size_t valid_stride, const float2 *tw, const float2 *wl,
+                                       const float *gain, void *out, size_t out_station_stride, int tile,
+                                       int frames, int n_station, hipStream_t s) {
+    const dim3 grid((unsigned)frames, (unsigned)n_station);
+    const dim3 block((unsigned)fft::MixedSizes<M, LOG2P>::kThreads);
+    hipLaunchKernelGGL((fengine_mixed_form_kernel<M, LOG2P, FORM>), grid, block, 0, s, in, station_pitch, valid,
+                       valid_stride, tw, wl, gain, out, out_station_stride, tile);
+    return hipGetLastError();
+}
+
+template <int M, int LOG2P>
+hipError_t launch_fengine_mixed_fp32_t(const int8_t *in, size_t station_pitch, const int32_t *valid,
+                                       size_t valid_stride, const float2 *tw, const float2 *wl, float2 *out,
+                                       size_t out_station_stride, int frames, int n_station, hipStream_t s) {
+    const dim3 grid((unsigned)frames, (unsigned)n_station);
+    const dim3 block((unsigned)fft::MixedSizes<M, LOG2P>::kThreads);
+    hipLaunchKernelGGL((fengine_mixed_fp32_kernel<M, LOG2P>), grid, block, 0, s, in, station_pitch, valid,
+                       valid_stride, tw, wl, out, out_station_stride);
+    return hipGetLastError();
+}
+
+// tile: channels of the spectrum tile, 16, 32 or n_freq (the plain order).
 template <int FORM>
-hipError_t launch_fengine_form_f(int log2_nf, const int8_t *in, size_t station_pitch,
+hipError_t launch_fengine_form_f(int n_freq, const int8_t *in, size_t station_pitch,
                                  const int32_t *valid, size_t valid_stride, const float2 *tw,
                                  const float2 *wl, const float *gain, void *out,
-                                 size_t out_station_stride, int tile_shift, int frames,
+                                 size_t out_station_stride, int tile, int frames,
                                  int n_station, hipStream_t s) {
-    switch (log2_nf) {
+    switch (n_freq) {
 #define CORR_FFORM_CASE(L)                                                                       \
-    case L:                                                                                      \
+    case 1 << L:                                                                                 \
         return launch_fengine_form_t<L, FORM>(in, station_pitch, valid, valid_stride, tw, wl,   \
-                                              gain, out, out_station_stride, tile_shift, frames, \
+                                              gain, out, out_station_stride,                    \
+                                              tile == 16 ? 4 : tile == 32 ? 5 : L, frames,      \
                                               n_station, s);
         CORR_FFORM_CASE(9)
         CORR_FFORM_CASE(10)
@@ -259,6 +328,15 @@ hipError_t launch_fengine_form_f(int log2_nf, const int8_t *in, size_t station_p
         CORR_FFORM_CASE(13)
         CORR_FFORM_CASE(14)
 #undef CORR_FFORM_CASE
+#define CORR_FMIXED_CASE(M, L)                                                                            \
+    case M << L:                                                                                          \
+        return launch_fengine_mixed_form_t<M, L, FORM>(in, station_pitch, valid, valid_stride, tw, wl, gain, \
+                                                       out, out_station_stride, tile, frames, n_station, s);
+        CORR_FMIXED_CASE(3, 11)
+        CORR_FMIXED_CASE(5, 11)
+        CORR_FMIXED_CASE(3, 12)
+        CORR_FMIXED_CASE(7, 11)
+#undef CORR_FMIXED_CASE
     default:
         return hipErrorInvalidValue;
     }
@@ -329,10 +407,56 @@ hipError_t launch_fengine_fp32_t(const int8_t *in, size_t station_pitch, const i
 
 }  // namespace
 
+hipError_t launch_fengine_fp32(int n_freq, const int8_t *in, size_t station_pitch,
+                               const int32_t *valid, size_t valid_stride, const float2 *tw,
+                               const float2 *wl, float2 *out, size_t out_station_stride,
+                               int frames, int n_station, hipStream_t s) {
+    if (frames <= 0 || n_station <= 0) return hipSuccess;
+    switch (n_freq) {
+#define CORR_FENGINE_CASE(L)                                                                      \
+    case 1 << L:                                                                                  \
+        return launch_fengine_fp32_t<L>(in, station_pitch, valid, valid_stride, tw, wl, out, \
+                                        out_station_stride, frames, n_station, s);
+        CORR_FENGINE_CASE(9)
+        CORR_FENGINE_CASE(10)
+        CORR_FENGINE_CASE(11)
+        CORR_FENGINE_CASE(12)
+        CORR_FENGINE_CASE(13)
+        CORR_FENGINE_CASE(14)
+#undef CORR_FENGINE_CASE
+#define CORR_FMIXED_CASE(M, L)                                                                                \
+    case M << L:                                                                                              \
+        return launch_fengine_mixed_fp32_t<M, L>(in, station_pitch, valid, valid_stride, tw, wl, out, \
+                                                 out_station_stride, frames, n_station, s);
+        CORR_FMIXED_CASE(3, 11)
+        CORR_FMIXED_CASE(5, 11)
+        CORR_FMIXED_CASE(3, 12)
+        CORR_FMIXED_CASE(7, 11)
+#undef CORR_FMIXED_CASE
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_unpack4(const int8_t *in, size_t in_pitch, int8_t *out, size_t out_pitch, int frames,
+                          int n_freq, int n_station, hipStream_t s) {
+    if (frames <= 0 || n_station <= 0) return hipSuccess;
+    // a packed frame is n_freq bytes, n_freq/4 words; pitches are whole words
+    if (n_freq % 4 != 0 || in_pitch % 4 != 0 || out_pitch % 8 != 0 ||
+        in_pitch < (size_t)frames * (size_t)n_freq || out_pitch < (size_t)frames * 2u * (size_t)n_freq)
+        return hipErrorInvalidValue;
+    const size_t words = (size_t)frames * (size_t)n_freq / 4u;
+    const dim3 grid((unsigned)((words + kUnpackBlock - 1) / kUnpackBlock), (unsigned)n_station);
+    hipLaunchKernelGGL(unpack4_kernel, grid, dim3(kUnpackBlock), 0, s, reinterpret_cast<const uint32_t *>(in),
+                       in_pitch / 4u, reinterpret_cast<uint64_t *>(out), out_pitch / 8u, words);
+    return hipGetLastError();
+}
+
 hipError_t launch_twiddles(float2 *tw, float2 *wl, int n_freq, hipStream_t s) {
     if (n_freq <= 0) return hipErrorInvalidValue;
+    const unsigned p = (unsigned)(fft_twiddle_elems(n_freq) - n_freq);
     hipLaunchKernelGGL(twiddle_kernel, dim3((unsigned)((n_freq + kTwiddleBlock - 1) / kTwiddleBlock)),
-                       dim3(kTwiddleBlock), 0, s, tw, wl, (unsigned)n_freq);
+                       dim3(kTwiddleBlock), 0, s, tw, wl, (unsigned)n_freq, p);
     return hipGetLastError();
 }
 
@@ -343,28 +467,6 @@ hipError_t launch_validity(const uint8_t *mask, size_t mask_pitch, int bit0, int
     hipLaunchKernelGGL(validity_kernel, grid, dim3(kValidityBlock), 0, s, mask, mask_pitch, bit0,
                        frames, valid, valid_stride);
     return hipGetLastError();
-}
-
-hipError_t launch_fengine_fp32(int log2_nf, const int8_t *in, size_t station_pitch,
-                               const int32_t *valid, size_t valid_stride, const float2 *tw,
-                               const float2 *wl, float2 *out, size_t out_station_stride,
-                               int frames, int n_station, hipStream_t s) {
-    if (frames <= 0 || n_station <= 0) return hipSuccess;
-    switch (log2_nf) {
-#define CORR_FENGINE_CASE(L)                                                              \
-    case L:                                                                               \
-        return launch_fengine_fp32_t<L>(in, station_pitch, valid, valid_stride, tw, wl, out, \
-                                        out_station_stride, frames, n_station, s);
-        CORR_FENGINE_CASE(9)
-        CORR_FENGINE_CASE(10)
-        CORR_FENGINE_CASE(11)
-        CORR_FENGINE_CASE(12)
-        CORR_FENGINE_CASE(13)
-        CORR_FENGINE_CASE(14)
-#undef CORR_FENGINE_CASE
-    default:
-        return hipErrorInvalidValue;
-    }
 }
 
 hipError_t launch_xengine_fp32(const float2 *y, size_t y_station_stride, int n_freq, int frames,
@@ -390,20 +492,20 @@ hipError_t launch_xengine_fp32(const float2 *y, size_t y_station_stride, int n_f
     }
 }
 
-hipError_t launch_fengine_form(int form, int log2_nf, const int8_t *in, size_t station_pitch,
+hipError_t launch_fengine_form(int form, int n_freq, const int8_t *in, size_t station_pitch,
                                const int32_t *valid, size_t valid_stride, const float2 *tw,
                                const float2 *wl, const float *gain, void *out,
                                size_t out_station_stride, int tile, int frames, int n_station,
                                hipStream_t s) {
     if (frames <= 0 || n_station <= 0) return hipSuccess;
-    if (tile != 16 && tile != 32 && tile != (1 << log2_nf)) return hipErrorInvalidValue;
-    const int shift = tile == 16 ? 4 : tile == 32 ? 5 : log2_nf;  // a tile of n_freq: [station][frame][channel]
+    // a tile of n_freq channels is the plain order [station][frame][channel]
+    if (tile != 16 && tile != 32 && tile != n_freq) return hipErrorInvalidValue;
     if (form == 1)
-        return launch_fengine_form_f<1>(log2_nf, in, station_pitch, valid, valid_stride, tw, wl, gain,
-                                        out, out_station_stride, shift, frames, n_station, s);
+        return launch_fengine_form_f<1>(n_freq, in, station_pitch, valid, valid_stride, tw, wl, gain,
+                                        out, out_station_stride, tile, frames, n_station, s);
     if (form == 2)
-        return launch_fengine_form_f<2>(log2_nf, in, station_pitch, valid, valid_stride, tw, wl, gain,
-                                        out, out_station_stride, shift, frames, n_station, s);
+        return launch_fengine_form_f<2>(n_freq, in, station_pitch, valid, valid_stride, tw, wl, gain,
+                                        out, out_station_stride, tile, frames, n_station, s);
     return hipErrorInvalidValue;
 }
 

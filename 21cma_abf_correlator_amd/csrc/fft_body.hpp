@@ -268,6 +268,28 @@ CORR_HD CORR_FI void expand_validity(int f, int st, int frames, const uint8_t *m
     if (f < frames) valid[(size_t)st * valid_stride + (size_t)f] = mask_bit(mask, mask_pitch, bit0, f, st);
 }
 
+// 4-bit staging (corrx_create_packed): a 32-bit word of packed samples holds
+// samples 8i..8i+7, sample 2m in bits 0-3 and 2m+1 in bits 4-7 of byte m, two's
+// complement. Returned as eight int8 bytes, sample 8i in the low byte: the
+// staging layout of the same samples at 8 bits.
+CORR_HD CORR_FI uint64_t unpack4_word(uint32_t w) {
+    uint64_t out = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t n = (w >> (4 * k)) & 15u;
+        const uint32_t b = (n ^ 8u) - 8u;  // the nibble sign-extended to 32 bits
+        out |= (uint64_t)(b & 0xFFu) << (8 * k);
+    }
+    return out;
+}
+
+// Word i of station st's first `words` packed words (pitches in words; the
+// packed and unpacked pitches hold the same number of samples).
+CORR_HD CORR_FI void unpack4(size_t i, int st, const uint32_t *in, size_t in_pitch, uint64_t *out,
+                            size_t out_pitch, size_t words) {
+    if (i < words) out[(size_t)st * out_pitch + i] = unpack4_word(in[(size_t)st * in_pitch + i]);
+}
+
 // Thread j's 16 input values of frame f of station st, z[j + r*N/16] for
 // r < 16, converted exactly from int8 pairs.
 template <int LOG2N>
@@ -699,6 +721,229 @@ CORR_HD CORR_FI void fengine_frame(int j, int f, int st, float2 *lds, Sync &sync
         complex_fft_block<LOG2N>(j, f, st, lds, sync, in, station_pitch, tw);
         r2c_epilogue<N>(j, lds, wl, emit);
     }
+}
+
+// ---------------------------------------------------------------------------
+// n_freq = M * 2^a with an odd factor M in {3, 5, 7}: the lengths 6144, 10240,
+// 12288 and 14336 that the reference accepts (every multiple of 2048,
+// corr-cuda/src/corr_cuda.cpp:500-509) besides the powers of two.
+//
+// The N = M*P point complex transform (P = 2^LOG2P) takes one radix-M stage
+// by decimation in frequency, then M independent P-point Stockham transforms
+// (the passes above), all in LDS:
+//   n = n1 + P n2, k = M k1 + k2 (n1, k1 < P; n2, k2 < M)
+//   y_k2[n1] = W_N^(n1 k2) sum_n2 z[n1 + P n2] W_M^(n2 k2)
+//   Z[M k1 + k2] = sum_n1 y_k2[n1] W_P^(n1 k1)
+// Sub-transform k2 lives in its own LDS span and runs on lanes
+// [k2 P/16, (k2+1) P/16), whole waves, with the passes' own lane numbering.
+// The R2C epilogue then reads Z[k] at (k mod M, k div M).
+// ---------------------------------------------------------------------------
+template <int M, int LOG2P>
+struct MixedSizes {
+    static_assert(M == 3 || M == 5 || M == 7, "odd factor 3, 5 or 7");
+    static constexpr int P = 1 << LOG2P;
+    static constexpr int N = M * P;
+    static constexpr int kThreads = N / 16;
+    static constexpr int kSubThreads = P / 16;
+    // One sub-transform's span: the P-point exchange buffer plus 8 float2, so
+    // that equal offsets in neighbouring spans fall in different LDS banks.
+    static constexpr int kSub = Sizes<P>::kLds + 8;
+    static constexpr int kLds = M * kSub;  // float2 elements
+};
+
+// cos(2 pi m/M) and sin(2 pi m/M) for 0 < m <= (M-1)/2, to 20 digits.
+template <int M>
+CORR_HD constexpr float cos_m(int m) {
+    return M == 3   ? -0.5f
+           : M == 5 ? (m == 1 ? 0.30901699437494742410f : -0.80901699437494742410f)
+                    : (m == 1   ? 0.62348980185873353053f
+                       : m == 2 ? -0.22252093395631440429f
+                                : -0.90096886790241912624f);
+}
+template <int M>
+CORR_HD constexpr float sin_m(int m) {
+    return M == 3   ? 0.86602540378443864676f
+           : M == 5 ? (m == 1 ? 0.95105651629515357212f : 0.58778525229247312917f)
+                    : (m == 1   ? 0.78183148246802980871f
+                       : m == 2 ? 0.97492791218182360702f
+                                : 0.43388373911755812048f);
+}
+
+// In-place DFT of odd length M, natural order: with t_n = a_n + a_(M-n) and
+// u_n = a_n - a_(M-n) (0 < n <= H = (M-1)/2),
+//   X_0 = a_0 + sum t_n,
+//   X_k = C_k - i S_k, X_(M-k) = C_k + i S_k (0 < k <= H),
+//   C_k = a_0 + sum_n t_n cos(2 pi nk/M), S_k = sum_n u_n sin(2 pi nk/M),
+// with nk reduced mod M into the first half by cos symmetry and sin antisymmetry.
+template <int M>
+CORR_HD CORR_FI void dft_odd(float2 (&a)[M]) {
+    constexpr int H = (M - 1) / 2;
+    float2 t[H + 1], u[H + 1];
+#pragma unroll
+    for (int n = 1; n <= H; ++n) {
+        t[n] = cadd(a[n], a[M - n]);
+        u[n] = csub(a[n], a[M - n]);
+    }
+    float2 x0 = a[0];
+#pragma unroll
+    for (int n = 1; n <= H; ++n) x0 = cadd(x0, t[n]);
+    float2 out[M];
+    out[0] = x0;
+#pragma unroll
+    for (int k = 1; k <= H; ++k) {
+        float2 c = a[0], s = make_float2(0.0f, 0.0f);
+#pragma unroll
+        for (int n = 1; n <= H; ++n) {
+            const int r = (n * k) % M;
+            const int m = r <= H ? r : M - r;
+            const float cs = cos_m<M>(m), sn = r <= H ? sin_m<M>(m) : -sin_m<M>(m);
+            c = make_float2(c.x + t[n].x * cs, c.y + t[n].y * cs);
+            s = make_float2(s.x + u[n].x * sn, s.y + u[n].y * sn);
+        }
+        out[k] = make_float2(c.x + s.y, c.y - s.x);
+        out[M - k] = make_float2(c.x - s.y, c.y + s.x);
+    }
+#pragma unroll
+    for (int k = 0; k < M; ++k) a[k] = out[k];
+}
+
+// The radix-M stage: lane j takes n1 = j + i*N/16 (< P), reads the M inputs
+// z[n1 + P n2] of the frame (int8 pairs, exact in float), and writes
+// y_k2[n1] = DFT_M(...)[k2] * W_N^(n1 k2) at offset n1 of span k2 (plain
+// layout). twn[m] = exp(-2 pi i m/N), m < N; n1 k2 < N.
+template <int M, int LOG2P>
+CORR_HD CORR_FI void mixed_first_stage(int j, const char2 *src, float2 *lds, const float2 *twn) {
+    using S = MixedSizes<M, LOG2P>;
+    constexpr int NI = (S::P + S::kThreads - 1) / S::kThreads;
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+        const int n1 = j + i * S::kThreads;
+        if (n1 < S::P) {
+            float2 a[M];
+#pragma unroll
+            for (int n2 = 0; n2 < M; ++n2) {
+                const char2 c = src[(unsigned)(n1 + n2 * S::P)];
+                a[n2] = make_float2((float)c.x, (float)c.y);
+            }
+            dft_odd<M>(a);
+            lds[n1] = a[0];
+#pragma unroll
+            for (int k2 = 1; k2 < M; ++k2) lds[k2 * S::kSub + n1] = cmul(a[k2], twn[(unsigned)(n1 * k2)]);
+        }
+    }
+}
+
+// Z of one frame into the M spans (span k2 holds Z[M k1 + k2] at k1, plain
+// layout); ends with a barrier. twp[m] = exp(-2 pi i m/P), m < P.
+template <int M, int LOG2P, class Sync>
+CORR_HD CORR_FI void mixed_fft_block(int j, const char2 *src, float2 *lds, Sync &sync, const float2 *twn,
+                                    const float2 *twp) {
+    using S = MixedSizes<M, LOG2P>;
+    mixed_first_stage<M, LOG2P>(j, src, lds, twn);
+    sync();
+    const int s = j / S::kSubThreads, jl = j % S::kSubThreads;
+    float2 *sub = lds + s * S::kSub;
+    float2 v[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = sub[jl + r * S::kSubThreads];
+    sync();
+    fft_from_values<LOG2P>(jl, v, sub, sync, twp);
+}
+
+// Z[k] of the mixed layout.
+template <int M, int LOG2P>
+CORR_HD CORR_FI float2 mixed_z(const float2 *lds, int k) {
+    return lds[(unsigned)k % (unsigned)M * (unsigned)MixedSizes<M, LOG2P>::kSub + (unsigned)k / (unsigned)M];
+}
+
+// The R2C bins from a = Z[k], b = Z[N-k] (N-k taken mod N) and w = W^k:
+// X[k] = E + w O, X[N-k] = conj(E - w O).
+CORR_HD CORR_FI void r2c_from(float2 a, float2 b, float2 w, float2 &xk, float2 &xm) {
+    const float er = 0.5f * (a.x + b.x), ei = 0.5f * (a.y - b.y);
+    const float orr = 0.5f * (a.y + b.y), oi = -0.5f * (a.x - b.x);
+    const float tr = w.x * orr - w.y * oi, ti = w.x * oi + w.y * orr;
+    xk = make_float2(er + tr, ei + ti);
+    xm = make_float2(er - tr, ti - ei);
+}
+
+// The R2C epilogue over the mixed layout, bins in the order and lane
+// assignment of r2c_epilogue (k = j + q*N/16 with its mirror; bins 0, N/2).
+template <int M, int LOG2P, class Emit>
+CORR_HD CORR_FI void r2c_epilogue_mixed(int j, const float2 *lds, const float2 *wl, Emit &&emit) {
+    constexpr int N = M << LOG2P;
+    constexpr int NT = N / 16;
+    const float2 wj = wl[(unsigned)j];
+    emit.begin();
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int k = j + q * NT;
+        float2 xk, xm;
+        if (k == 0) {
+            const float2 z0 = mixed_z<M, LOG2P>(lds, 0);
+            r2c_from(z0, z0, wl[0], xk, xm);
+            emit.at(0, xk);
+        } else {
+            const float2 w = q == 0 ? wj : cmul(wj, make_float2(cos32(q), -cos32((q + 24) & 31)));
+            r2c_from(mixed_z<M, LOG2P>(lds, k), mixed_z<M, LOG2P>(lds, N - k), w, xk, xm);
+            emit.pair(q, xk, xm);
+        }
+    }
+    if (j == 0) {
+        const float2 zh = mixed_z<M, LOG2P>(lds, N / 2);
+        float2 xk, xm;
+        r2c_from(zh, zh, wl[(unsigned)(N / 2)], xk, xm);
+        emit.at(N / 2, xk);
+    }
+}
+
+// One frame of one station, n_freq = M * 2^LOG2P; tw holds exp(-2 pi i m/N)
+// for m < N followed by exp(-2 pi i m/P) for m < P.
+template <int M, int LOG2P, class Sync, class Emit>
+CORR_HD CORR_FI void fengine_mixed_frame(int j, int f, int st, float2 *lds, Sync &sync, const int8_t *in,
+                                         size_t station_pitch, const int32_t *valid, size_t valid_stride,
+                                         const float2 *tw, const float2 *wl, Emit &&emit) {
+    constexpr int N = M << LOG2P;
+    if (!frame_present(valid, valid_stride, f, st)) {
+        emit_zero_frame<N>(j, emit);
+        return;
+    }
+    const char2 *src =
+        reinterpret_cast<const char2 *>(in + (size_t)st * station_pitch + (size_t)f * (size_t)(2 * N));
+    mixed_fft_block<M, LOG2P>(j, src, lds, sync, tw, tw + N);
+    r2c_epilogue_mixed<M, LOG2P>(j, lds, wl, emit);
+}
+
+// Complex64 output [station][frame][N], as fengine_fp32_block.
+template <int M, int LOG2P, class Sync>
+CORR_HD CORR_FI void fengine_mixed_fp32_block(int j, int f, int st, float2 *lds, Sync &sync, const int8_t *in,
+                                             size_t station_pitch, const int32_t *valid, size_t valid_stride,
+                                             const float2 *tw, const float2 *wl, float2 *out,
+                                             size_t out_station_stride) {
+    constexpr int N = M << LOG2P;
+    Fp32Emit<N> e{out + (size_t)st * out_station_stride + (size_t)f * (size_t)N, j};
+    fengine_mixed_frame<M, LOG2P>(j, f, st, lds, sync, in, station_pitch, valid, valid_stride, tw, wl, e);
+}
+
+// The FP16 / int8 forms, as fengine_form_block, with the spectrum tile given
+// in channels: 16, 32 or N (the plain order). For N the emitter's tile shift
+// is 15, beyond every bin, so that off(k) = k.
+template <int M, int LOG2P, int FORM, class Sync>
+CORR_HD CORR_FI void fengine_mixed_form_block(int j, int f, int st, float2 *lds, Sync &sync, const int8_t *in,
+                                             size_t station_pitch, const int32_t *valid, size_t valid_stride,
+                                             const float2 *tw, const float2 *wl, const float *gain, void *out,
+                                             size_t out_station_stride, int tile) {
+    constexpr int N = M << LOG2P;
+    static_assert(N < (1 << 15), "the plain order's tile shift");
+    using Emit = FormEmit<N, FORM>;
+    const unsigned shift = tile == 16 ? 4u : tile == 32 ? 5u : 15u;
+    Emit e;
+    e.dst = static_cast<typename Emit::word_t *>(out) + (size_t)st * out_station_stride + (size_t)f * (size_t)tile;
+    e.g = gain + (size_t)st * N;
+    e.shift = shift;
+    e.mask = (1u << shift) - 1u;
+    e.tile_stride = tile == N ? 0u : (unsigned)(out_station_stride / N) * (unsigned)tile;  // fcap * T
+    e.j = j;
+    fengine_mixed_frame<M, LOG2P>(j, f, st, lds, sync, in, station_pitch, valid, valid_stride, tw, wl, e);
 }
 
 // Calibration: power[st*nf + c] += sum over the frames present of |Y|^2, in

@@ -258,14 +258,16 @@ def batch_c5(corr, hip, args, fp64):
             "subarrays": rep}
 
 
-def abi_end_to_end(corr, args, pool_host, fp64):
+def abi_end_to_end(corr, args, pool_host, fp64, sample_bits=8):
     """The drop-in path end to end: frames in pinned staging (as the receiver
     leaves them) -> corr_row_accumulate (H2D + F + X, FP32 form) per slot ->
     corr_row_end -> one corr_rows_fetch of the row stack to host memory.
     Geometry of the reference's 24-input nf 16384 YAML: 256 frames per slot,
     11 slots, rows of 2048 frames. The staging is written once before timing;
     each released slot gets its mask bits back (the library clears them on
-    release, the receiver sets them). PCIe-inclusive: not the headline value."""
+    release, the receiver sets them). PCIe-inclusive: not the headline value.
+    sample_bits 4: a corrx_create_packed context, the same frames requantized
+    to 4 bits (clip(rint(x/4), -8, 7)) and packed two per byte."""
     import numpy as np
     N, P, L = pool_host.shape
     nf = L // 2
@@ -273,11 +275,16 @@ def abi_end_to_end(corr, args, pool_host, fp64):
         return {"skipped": "the drop-in ABI takes n_freq in multiples of 2048 (its packet rule)"}
     fps, slots, n_cum, rows = 256, 11, 2048, 4
     cfg = corr.make_config(N, nf, fps, fps, block_slots=slots, rows_per_fetch=rows)
-    with corr.Correlator(cfg) as c:
+    if sample_bits == 4:
+        pool_host = np.clip(np.rint(pool_host / 4.0), -8, 7).astype(np.int8)
+        staged = corr.pack4(pool_host)
+    else:
+        staged = pool_host
+    with corr.Correlator(cfg, sample_bits=sample_bits) as c:
         B = c.n_baseline
         for g in range(N):
             for sl in range(slots):
-                c.slot_frames(g, sl)[:] = pool_host[g, (sl * fps + np.arange(fps)) % P]
+                c.slot_frames(g, sl)[:] = staged[g, (sl * fps + np.arange(fps)) % P]
         masks = [[c.slot_mask(g, sl) for sl in range(slots)] for g in range(N)]
         per_row = n_cum // fps
         out = np.empty(c.fetch_bytes() // 8, dtype=np.complex64)
@@ -305,10 +312,11 @@ def abi_end_to_end(corr, args, pool_host, fp64):
         dt = time.perf_counter() - t0
         frames = rows * n_cum
         res = {"workload": f"drop-in ABI, FP32 form, {N} in, nf {nf}, {fps} frames per slot, {slots} slots, "
-                           f"{rows} rows of {n_cum} frames, pinned staging -> visibilities in host memory",
+                           f"{rows} rows of {n_cum} frames, {sample_bits}-bit samples in pinned staging "
+                           f"-> visibilities in host memory",
                "ms_per_row": round(dt / rows * 1e3, 3),
                "Msamples_per_s_per_input": round(2 * nf * frames / dt / 1e6, 1),
-               "h2d_GB_per_s": round(N * frames * L / dt / 1e9, 2),
+               "h2d_GB_per_s": round(N * frames * L * sample_bits / 8 / dt / 1e9, 2),
                "realtime_factor": round(2 * nf * frames / dt / 800e6, 2)}
         if fp64:
             V = out.reshape(rows, B, nf)
@@ -517,6 +525,7 @@ def main():
 
     batch = None if args.no_batch else batch_c5(corr, hip, args, not args.no_fp64)
     abi = None if args.no_abi else abi_end_to_end(corr, args, pool_host, not args.no_fp64)
+    abi4 = None if args.no_abi else abi_end_to_end(corr, args, pool_host, not args.no_fp64, sample_bits=4)
 
     cpu = None
     if args.cpu_seconds > 0:
@@ -546,6 +555,7 @@ def main():
         "cpu_baseline": cpu,
         "batch": batch,
         "abi_end_to_end": abi,
+        "abi_end_to_end_4bit": abi4,
     }
     print(json.dumps(out))
 

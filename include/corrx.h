@@ -34,7 +34,7 @@ extern "C" {
 
 typedef struct corrx_shape {
     int32_t n_station;    /* inputs, >= 1                                    */
-    int32_t n_freq;       /* retained channels; power of two in [512, 16384] */
+    int32_t n_freq;       /* retained channels: 2^k in [512, 16384], or 6144, 10240, 12288, 14336 */
     int32_t chunk_frames; /* most frames one fengine/xengine call handles    */
     int32_t form;         /* CORRX_FORM_*                                    */
 } corrx_shape;
@@ -156,6 +156,22 @@ corr_status corrx_scale(void *d_acc, size_t n_complex, uint64_t frames, void *st
    bits, real in the low byte (INT8), each the spectrum times the gain of its
    (input, channel). station_stride_elems = chunk_frames * n_freq. */
 corr_status corrx_spectra(void *engine, void **d_spectra, size_t *station_stride_elems);
+
+/* A drop-in context (corr_cuda.h) whose staging holds samples of `sample_bits`
+   bits: 8 is exactly corr_create; 4 packs two 4-bit two's complement samples
+   per byte, sample 2m of a frame in bits 0-3 and sample 2m+1 in bits 4-7 of
+   byte m, values -8..7. With 4 bits the staging's frame_stride is n_freq bytes
+   (corr_staging_info reports the packed geometry) and each chunk's
+   host-to-device copy moves half the bytes. On the GPU the chunk is unpacked to
+   int8 and transformed by the F-engine of corr_create, so the rows are
+   byte-identical to corr_create's on the same samples; every other call
+   behaves as for corr_create. The caller packs and clips; values outside -8..7
+   cannot be represented.
+   No counterpart in the reference's ABI: it is the reference's proposed next
+   step "4-bit staging packing" (HANDOFF.md:1437-1439, 1863-1864), for receive
+   paths whose samples fit 4 bits and whose host link bounds the rate.
+   Errors as corr_create, plus CORR_ERR_ARG for sample_bits other than 8 or 4. */
+corr_status corrx_create_packed(const corr_config *cfg, int32_t sample_bits, void **out_ctx);
 
 #ifdef __cplusplus
 }

@@ -30,7 +30,9 @@
  * Pinning (how each function is checked; see tests/test_oracle.py):
  *   orc_pairs         abi_smoke.c:86-93 known answer (4 stations), tests/abi.rs
  *                     counts for N=1..24, closed-form index j*N - j(j-1)/2 + (k-j)
- *   orc_fft_r2c_f64   brute-force DFT (tiny L), numpy.fft.rfft at L=2048..32768,
+ *   orc_fft_r2c_f64   brute-force DFT (tiny L, powers of two and odd parts 3-15),
+ *                     numpy.fft.rfft at L=2048..32768 and at the non-power-of-two
+ *                     L = 2 * {6144, 10240, 12288, 14336, 18432},
  *                     impulse -> flat, pure tone -> single bin, Parseval
  *   orc_fx_f64        numpy FP64 restatement of gen_golden.py on committed
  *                     fixtures; autocorrelation imaginary part ~ 0; delay test
@@ -172,12 +174,59 @@ static void fft_c64(double *re, double *im, int n) {
     }
 }
 
+/* Any length n = m * 2^a: decimation in time by the odd part m. The m
+   sub-sequences x[m t + r] (t < 2^a) are transformed by fft_c64, then
+   X[k] = sum_r W_n^(r k) X_r[k mod 2^a], a direct m-term sum per bin with the
+   phase reduced exactly, (r k) mod n, into a table of W_n built in double.
+   The reference accepts every n_freq that is a multiple of 2048
+   (corr-cuda/src/corr_cuda.cpp:500-509), so L = 2 n_freq has an odd part of
+   any size; the work is O(n (m + log n)). */
+static void fft_any_c64(double *re, double *im, int n) {
+    const int p = n & -n;
+    const int m = n / p;
+    if (m == 1) {
+        fft_c64(re, im, n);
+        return;
+    }
+    double *sr = (double *)malloc(sizeof(double) * (size_t)n);
+    double *si = (double *)malloc(sizeof(double) * (size_t)n);
+    double *wr = (double *)malloc(sizeof(double) * (size_t)n);
+    double *wi = (double *)malloc(sizeof(double) * (size_t)n);
+    for (int r = 0; r < m; ++r)
+        for (int t = 0; t < p; ++t) {
+            sr[(size_t)r * p + t] = re[(size_t)m * t + r];
+            si[(size_t)r * p + t] = im[(size_t)m * t + r];
+        }
+    if (p > 1)
+        for (int r = 0; r < m; ++r) fft_c64(sr + (size_t)r * p, si + (size_t)r * p, p);
+    for (int q = 0; q < n; ++q) {
+        const double ang = -2.0 * ORC_PI * (double)q / (double)n;
+        wr[q] = cos(ang);
+        wi[q] = sin(ang);
+    }
+    for (int k = 0; k < n; ++k) {
+        double ar = 0.0, ai = 0.0;
+        for (int r = 0; r < m; ++r) {
+            const int q = (int)(((long long)r * k) % n);
+            const double xr = sr[(size_t)r * p + (k & (p - 1))], xi = si[(size_t)r * p + (k & (p - 1))];
+            ar += xr * wr[q] - xi * wi[q];
+            ai += xr * wi[q] + xi * wr[q];
+        }
+        re[k] = ar;
+        im[k] = ai;
+    }
+    free(sr);
+    free(si);
+    free(wr);
+    free(wi);
+}
+
 /* x: L real doubles; out: (L/2) complex as interleaved (re, im); bins 0..L/2-1. */
 void orc_fft_r2c_f64(const double *x, int L, double *out) {
     double *re = (double *)malloc(sizeof(double) * (size_t)L);
     double *im = (double *)malloc(sizeof(double) * (size_t)L);
     for (int i = 0; i < L; ++i) { re[i] = x[i]; im[i] = 0.0; }
-    fft_c64(re, im, L);
+    fft_any_c64(re, im, L);
     for (int k = 0; k < L / 2; ++k) { out[2 * k] = re[k]; out[2 * k + 1] = im[k]; }
     free(re);
     free(im);
@@ -332,6 +381,7 @@ int orc_fx_f32(const int8_t *x, const uint8_t *valid, int N, int T, int L,
                int chunk_frames, uint64_t frames_div, float *V) {
     const int nf = L / 2;
     const int B = orc_n_baseline(N);
+    if (nf < 1 || (nf & (nf - 1)) != 0) return -1; /* its radix-2 FFT: power-of-two n_freq */
     cf32 *tw = (cf32 *)malloc(sizeof(cf32) * (size_t)(nf / 2 > 0 ? nf / 2 : 1));
     cf32 *wl = (cf32 *)malloc(sizeof(cf32) * (size_t)nf);
     for (int m = 0; m < nf / 2; ++m) {

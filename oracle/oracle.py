@@ -121,6 +121,8 @@ def fx_f32(x, chunk_frames=256, frames_div=None, valid=None):
     V = np.empty(2 * B * (L // 2), dtype=np.float32)
     keep, vp = _valid_ptr(valid, N, T)
     threads = lib().orc_fx_f32(x, vp, N, T, L, int(chunk_frames), int(frames_div), V)
+    if threads < 0:
+        raise ValueError(f"fx_f32: n_freq {L // 2} is not a power of two (its radix-2 FFT)")
     return V.view(np.complex64).reshape(B, L // 2), threads
 
 

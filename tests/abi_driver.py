@@ -13,15 +13,18 @@ def frame_is_missing(row, frame_in_row):
     return ((row * 3 + frame_in_row) % MISSING_EVERY) == MISSING_PHASE
 
 
-def drive(corr, cfg, x, rows, n_cum, mode=ALL_VALID, lib=None, missing=None, stats=None):
+def drive(corr, cfg, x, rows, n_cum, mode=ALL_VALID, lib=None, missing=None, stats=None, sample_bits=8):
     """x: int8 (rows, N, n_cum, L). Returns complex64 (rows, B, nf) from one fetch.
     `missing(row, frame_in_row)` overrides the missing-frame pattern of the
     HOST_ZEROED / MASK_MARKED modes; `stats`, a list, receives the context's
-    device statistics before it is destroyed."""
+    device statistics before it is destroyed. sample_bits 4 drives a context with
+    4-bit packed staging (x must lie in [-8, 7])."""
     is_missing = missing if missing is not None else frame_is_missing
     fps = cfg.frames_per_slot
     N = cfg.n_station
-    with corr.Correlator(cfg, lib=lib) as c:
+    if sample_bits == 4:
+        x = corr.pack4(x)
+    with corr.Correlator(cfg, lib=lib, sample_bits=sample_bits) as c:
         submits = 0
         for r in range(rows):
             for f0 in range(0, n_cum, fps):

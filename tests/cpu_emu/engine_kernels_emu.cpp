@@ -65,6 +65,34 @@ hipError_t fengine_form_t(const int8_t *in, size_t station_pitch, const int32_t 
     return hipSuccess;
 }
 
+template <int M, int LOG2P>
+hipError_t fengine_mixed_t(const int8_t *in, size_t station_pitch, const int32_t *valid, size_t valid_stride,
+                           const float2 *tw, const float2 *wl, float2 *out, size_t out_station_stride, int frames,
+                           int n_station) {
+    using S = fft::MixedSizes<M, LOG2P>;
+    run_grid(frames, n_station, S::kThreads, S::kLds * sizeof(float2),
+             [=](int j, int bx, int by, char *lds, HostSync &sync) {
+                 fft::fengine_mixed_fp32_block<M, LOG2P>(j, bx, by, reinterpret_cast<float2 *>(lds), sync, in,
+                                                         station_pitch, valid, valid_stride, tw, wl, out,
+                                                         out_station_stride);
+             });
+    return hipSuccess;
+}
+
+template <int M, int LOG2P, int FORM>
+hipError_t fengine_mixed_form_t(const int8_t *in, size_t station_pitch, const int32_t *valid, size_t valid_stride,
+                                const float2 *tw, const float2 *wl, const float *gain, void *out,
+                                size_t out_station_stride, int tile, int frames, int n_station) {
+    using S = fft::MixedSizes<M, LOG2P>;
+    run_grid(frames, n_station, S::kThreads, S::kLds * sizeof(float2),
+             [=](int j, int bx, int by, char *lds, HostSync &sync) {
+                 fft::fengine_mixed_form_block<M, LOG2P, FORM>(j, bx, by, reinterpret_cast<float2 *>(lds), sync,
+                                                               in, station_pitch, valid, valid_stride, tw, wl, gain,
+                                                               out, out_station_stride, tile);
+             });
+    return hipSuccess;
+}
+
 template <int FORM, int NRT, int CH>
 void xmfma_t(int n_station, const void *spectra, size_t ys, int nf, int spec_tile, int frames, void *integ) {
     const int n_tiles = nf / CH;
@@ -110,6 +138,11 @@ hipError_t launch_twiddles(float2 *tw, float2 *wl, int n_freq, hipStream_t) {
         tw[i] = make_float2((float)std::cos(a), (float)std::sin(a));
         wl[i] = make_float2((float)std::cos(b), (float)std::sin(b));
     }
+    const int p = fft_twiddle_elems(n_freq) - n_freq;  // the sub-transforms' table
+    for (int i = 0; i < p; ++i) {
+        const double a = -2.0 * M_PI * i / p;
+        tw[n_freq + i] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
     return hipSuccess;
 }
 
@@ -123,20 +156,41 @@ hipError_t launch_validity(const uint8_t *mask, size_t mask_pitch, int bit0, int
     return hipSuccess;
 }
 
-hipError_t launch_fengine_fp32(int log2_nf, const int8_t *in, size_t station_pitch,
+hipError_t launch_fengine_fp32(int n_freq, const int8_t *in, size_t station_pitch,
                                const int32_t *valid, size_t valid_stride, const float2 *tw,
                                const float2 *wl, float2 *out, size_t out_station_stride,
                                int frames, int n_station, hipStream_t) {
     if (frames <= 0 || n_station <= 0) return hipSuccess;
-    switch (log2_nf) {
+    switch (n_freq) {
 #define E(L)  \
-    case L: \
+    case 1 << L: \
         return fengine_t<L>(in, station_pitch, valid, valid_stride, tw, wl, out, out_station_stride, frames, n_station);
         E(9) E(10) E(11) E(12) E(13) E(14)
+#undef E
+#define E(M, L)                                                                                                   \
+    case M << L:                                                                                                  \
+        return fengine_mixed_t<M, L>(in, station_pitch, valid, valid_stride, tw, wl, out, out_station_stride,   \
+                                         frames, n_station);
+        E(3, 11) E(5, 11) E(3, 12) E(7, 11)
 #undef E
     default:
         return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_unpack4(const int8_t *in, size_t in_pitch, int8_t *out, size_t out_pitch, int frames,
+                          int n_freq, int n_station, hipStream_t) {
+    if (frames <= 0 || n_station <= 0) return hipSuccess;
+    if (n_freq % 4 != 0 || in_pitch % 4 != 0 || out_pitch % 8 != 0 ||
+        in_pitch < (size_t)frames * (size_t)n_freq || out_pitch < (size_t)frames * 2u * (size_t)n_freq)
+        return hipErrorInvalidValue;
+    const size_t words = (size_t)frames * (size_t)n_freq / 4u;
+    const size_t lanes = (words + 255) / 256 * 256;  // the device grid
+    for (int st = 0; st < n_station; ++st)
+        for (size_t i = 0; i < lanes; ++i)
+            fft::unpack4(i, st, reinterpret_cast<const uint32_t *>(in), in_pitch / 4u,
+                         reinterpret_cast<uint64_t *>(out), out_pitch / 8u, words);
+    return hipSuccess;
 }
 
 hipError_t launch_xengine_fp32(const float2 *y, size_t y_station_stride, int n_freq, int frames,
@@ -159,17 +213,27 @@ hipError_t launch_xengine_fp32(const float2 *y, size_t y_station_stride, int n_f
     return hipSuccess;
 }
 
-hipError_t launch_fengine_form(int form, int log2_nf, const int8_t *in, size_t station_pitch,
+hipError_t launch_fengine_form(int form, int n_freq, const int8_t *in, size_t station_pitch,
                                const int32_t *valid, size_t valid_stride, const float2 *tw,
                                const float2 *wl, const float *gain, void *out,
                                size_t out_station_stride, int tile, int frames, int n_station, hipStream_t) {
     if (frames <= 0 || n_station <= 0) return hipSuccess;
     if (form != 1 && form != 2) return hipErrorInvalidValue;
-    if (tile != 16 && tile != 32 && tile != (1 << log2_nf)) return hipErrorInvalidValue;
+    if (tile != 16 && tile != 32 && tile != n_freq) return hipErrorInvalidValue;
+    int log2_nf = 0;
+    while ((1 << log2_nf) < n_freq) ++log2_nf;
     const int sh = tile == 16 ? 4 : tile == 32 ? 5 : log2_nf;
-    switch (log2_nf) {
+    switch (n_freq) {
+#define E(M, L)                                                                                                 \
+    case M << L:                                                                                                \
+        return form == 1 ? fengine_mixed_form_t<M, L, 1>(in, station_pitch, valid, valid_stride, tw, wl, gain, out, \
+                                                         out_station_stride, tile, frames, n_station)            \
+                         : fengine_mixed_form_t<M, L, 2>(in, station_pitch, valid, valid_stride, tw, wl, gain, out, \
+                                                         out_station_stride, tile, frames, n_station);
+        E(3, 11) E(5, 11) E(3, 12) E(7, 11)
+#undef E
 #define E(L)                                                                                            \
-    case L:                                                                                             \
+    case 1 << L:                                                                                        \
         return form == 1 ? fengine_form_t<L, 1>(in, station_pitch, valid, valid_stride, tw, wl, gain, out, \
                                                 out_station_stride, sh, frames, n_station)              \
                          : fengine_form_t<L, 2>(in, station_pitch, valid, valid_stride, tw, wl, gain, out, \

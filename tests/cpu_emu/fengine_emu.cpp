@@ -204,9 +204,156 @@ static int check() {
     return bad;
 }
 
+// n_freq = M * 2^LOG2P: the mixed-radix body, FP32 against the oracle's FP64
+// FFT (any length), the forms against pack() of the FP32 bins at every tile.
+template <int M, int LOG2P, class Body>
+static void run_mixed(Body body) {
+    using S = MixedSizes<M, LOG2P>;
+    float2 *lds = new float2[S::kLds];
+    std::barrier<> bar(S::kThreads);
+    std::vector<std::thread> th;
+    th.reserve(S::kThreads);
+    for (int j = 0; j < S::kThreads; ++j)
+        th.emplace_back([=, &bar] {
+            HostSync sync{&bar};
+            body(j, lds, sync);
+        });
+    for (auto &t : th) t.join();
+    delete[] lds;
+}
+
+template <int M, int LOG2P>
+static int check_mixed() {
+    constexpr int P = 1 << LOG2P, N = M * P;
+    const int NST = 2, FR = 2, BIT0 = 3;
+    const size_t pitch = (size_t)FR * 2 * N + 32;
+    std::mt19937 rng(99 + N);
+    std::uniform_int_distribution<int> d(-128, 127);
+    int8_t *in = new int8_t[pitch * NST];
+    for (size_t i = 0; i < pitch * NST; ++i) in[i] = (int8_t)d(rng);
+    uint8_t mask[2] = {0xFF, (uint8_t)~(1u << (BIT0 + 1))};  // station 1, frame 1 missing
+    int32_t *valid = new int32_t[(size_t)NST * FR];
+    for (int st = 0; st < NST; ++st)
+        for (int f = 0; f < FR; ++f) expand_validity(f, st, FR, mask, 1, BIT0, valid, FR);
+    float2 *tw = new float2[N + P];
+    float2 *wl = new float2[N];
+    for (int m = 0; m < N; ++m) {
+        const double a = -2.0 * M_PI * m / N, b = -M_PI * m / N;
+        tw[m] = make_float2((float)std::cos(a), (float)std::sin(a));
+        wl[m] = make_float2((float)std::cos(b), (float)std::sin(b));
+    }
+    for (int m = 0; m < P; ++m) {
+        const double a = -2.0 * M_PI * m / P;
+        tw[N + m] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    const size_t ostride = (size_t)FR * N;
+    float2 *out = new float2[ostride * NST];
+    for (int st = 0; st < NST; ++st)
+        for (int f = 0; f < FR; ++f)
+            run_mixed<M, LOG2P>([=](int j, float2 *lds, HostSync &sync) {
+                fengine_mixed_fp32_block<M, LOG2P>(j, f, st, lds, sync, in, pitch, valid, FR, tw, wl, out, ostride);
+            });
+    int bad = 0;
+    double worst = 0.0;
+    std::vector<double> x(2 * N), ref(2 * N);
+    for (int st = 0; st < NST; ++st)
+        for (int f = 0; f < FR; ++f) {
+            const float2 *y = out + st * ostride + (size_t)f * N;
+            if (st == 1 && f == 1) {
+                for (int k = 0; k < N; ++k)
+                    if (y[k].x != 0.0f || y[k].y != 0.0f) ++bad;
+                continue;
+            }
+            for (int n = 0; n < 2 * N; ++n) x[n] = in[st * pitch + (size_t)f * 2 * N + n];
+            orc_fft_r2c_f64(x.data(), 2 * N, ref.data());
+            double num = 0, den = 0;
+            for (int k = 0; k < N; ++k) {
+                const double dr = y[k].x - ref[2 * k], di = y[k].y - ref[2 * k + 1];
+                num += dr * dr + di * di;
+                den += ref[2 * k] * ref[2 * k] + ref[2 * k + 1] * ref[2 * k + 1];
+            }
+            const double e = std::sqrt(num / den);
+            if (e > worst) worst = e;
+            if (!(e < 1e-5)) ++bad;
+        }
+    std::printf("N=%d (%d x %d): worst relative RMS %.3e, %d bad\n", N, M, P, worst, bad);
+    // forms at tiles of 16, 32 and N
+    float *g16 = new float[(size_t)NST * N], *g8 = new float[(size_t)NST * N];
+    std::uniform_int_distribution<int> e(-12, 2);
+    std::uniform_real_distribution<float> u(0.01f, 0.2f);
+    for (size_t i = 0; i < (size_t)NST * N; ++i) {
+        g16[i] = std::ldexp(1.0f, e(rng));
+        g8[i] = u(rng);
+    }
+    uint32_t *o16 = new uint32_t[ostride * NST];
+    uint16_t *o8 = new uint16_t[ostride * NST];
+    int fbad = 0;
+    for (int tile : {16, 32, N}) {
+        for (int st = 0; st < NST; ++st)
+            for (int f = 0; f < FR; ++f) {
+                run_mixed<M, LOG2P>([=](int j, float2 *lds, HostSync &sync) {
+                    fengine_mixed_form_block<M, LOG2P, 1>(j, f, st, lds, sync, in, pitch, valid, FR, tw, wl, g16,
+                                                          o16, ostride, tile);
+                });
+                run_mixed<M, LOG2P>([=](int j, float2 *lds, HostSync &sync) {
+                    fengine_mixed_form_block<M, LOG2P, 2>(j, f, st, lds, sync, in, pitch, valid, FR, tw, wl, g8,
+                                                          o8, ostride, tile);
+                });
+            }
+        for (int st = 0; st < NST; ++st)
+            for (int f = 0; f < FR; ++f)
+                for (int k = 0; k < N; ++k) {
+                    const float2 y = out[st * ostride + (size_t)f * N + k];
+                    const size_t at = corr::forms::spec_index(st, f, k, N, FR, tile);
+                    if (o16[at] != corr::forms::pack_fp16(y, g16[(size_t)st * N + k])) ++fbad;
+                    if (o8[at] != corr::forms::pack_int8(y, g8[(size_t)st * N + k])) ++fbad;
+                }
+    }
+    std::printf("N=%d forms (tiles of 16, 32 and N): %d mismatches\n", N, fbad);
+    delete[] g16;
+    delete[] g8;
+    delete[] o16;
+    delete[] o8;
+    delete[] in;
+    delete[] valid;
+    delete[] tw;
+    delete[] wl;
+    delete[] out;
+    return bad + fbad;
+}
+
+// unpack4_word against the definition of the 4-bit staging: every byte value
+// at every byte position of the word, and random words.
+static int check_unpack4() {
+    int bad = 0;
+    auto want = [](uint32_t w) {
+        uint64_t r = 0;
+        for (int m = 0; m < 4; ++m) {
+            const int byte = (int)((w >> (8 * m)) & 0xFFu);
+            const int lo = (byte & 15) >= 8 ? (byte & 15) - 16 : (byte & 15);
+            const int hi = (byte >> 4) >= 8 ? (byte >> 4) - 16 : (byte >> 4);
+            r |= (uint64_t)(uint8_t)(int8_t)lo << (16 * m);
+            r |= (uint64_t)(uint8_t)(int8_t)hi << (16 * m + 8);
+        }
+        return r;
+    };
+    for (int m = 0; m < 4; ++m)
+        for (uint32_t v = 0; v < 256; ++v)
+            if (unpack4_word(v << (8 * m)) != want(v << (8 * m))) ++bad;
+    std::mt19937 rng(3);
+    for (int i = 0; i < 100000; ++i) {
+        const uint32_t w = rng();
+        if (unpack4_word(w) != want(w)) ++bad;
+    }
+    std::printf("4-bit unpack: %d mismatches\n", bad);
+    return bad;
+}
+
 int main() {
     int bad = check_half();
     bad += check<9>() + check<10>() + check<11>() + check<12>() + check<13>() + check<14>();
+    bad += check_mixed<3, 11>() + check_mixed<5, 11>() + check_mixed<3, 12>() + check_mixed<7, 11>();
+    bad += check_unpack4();
     std::printf(bad ? "FAILED\n" : "OK\n");
     return bad ? 1 : 0;
 }

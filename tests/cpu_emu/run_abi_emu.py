@@ -42,6 +42,27 @@ def golden(L, N, nf, n_cum, fps, rows):
     return e < 1e-4 and em < 1e-4 and same
 
 
+def packed(L, N, nf, n_cum, fps, rows):
+    """4-bit staging (corrx_create_packed) against the 8-bit context on the same
+    samples in [-8, 7]: byte-identical rows, all valid and mask-marked, and half
+    the sample bytes copied."""
+    rng = np.random.default_rng(7)
+    x = rng.integers(-8, 8, size=(rows, N, n_cum, 2 * nf)).astype(np.int8)
+    cfg = corr.make_config(N, nf, fps, min(fps, 128), block_slots=3, rows_per_fetch=rows)
+    s8, s4 = [], []
+    g8 = drive(corr, cfg, x, rows, n_cum, ALL_VALID, lib=L, stats=s8)
+    g4 = drive(corr, cfg, x, rows, n_cum, ALL_VALID, lib=L, stats=s4, sample_bits=4)
+    m8 = drive(corr, cfg, x, rows, n_cum, MASK_MARKED, lib=L)
+    m4 = drive(corr, cfg, x, rows, n_cum, MASK_MARKED, lib=L, sample_bits=4)
+    same = g4.tobytes() == g8.tobytes() and m4.tobytes() == m8.tobytes()
+    masks = s8[0].transfers * N * ((fps + 7) // 8)
+    half = 2 * (s4[0].bytes_h2d - masks) == s8[0].bytes_h2d - masks
+    e = rel_rms(g4, np.stack([oracle.fx_f64(x[r], frames_div=n_cum) for r in range(rows)]))
+    print(f"packed N={N} nf={nf}: 4-bit == 8-bit {same}, half the sample bytes {half}, rel RMS {e:.3e}",
+          flush=True)
+    return same and half and e < 1e-4
+
+
 def ptr(a):
     return a.ctypes.data
 
@@ -206,6 +227,8 @@ def main():
     L = corr.declare(ctypes.CDLL(os.path.join(HERE, "_build", "libcorr_cuda_emu.so")))
     ok = timed(golden, L, 4, 2048, 90, 32, 2)
     ok = timed(golden, L, 5, 4096, 37, 16, 3) and ok
+    ok = timed(golden, L, 3, 6144, 20, 16, 1) and ok  # n_freq = 3 x 2048
+    ok = timed(packed, L, 3, 2048, 40, 16, 2) and ok
     ok = timed(skip, L) and ok
     ok = timed(forms, L, corr.FORM_FP16, 5, 512, 70, 32, 20) and ok
     ok = timed(forms, L, corr.FORM_INT8, 6, 512, 70, 32, 70) and ok

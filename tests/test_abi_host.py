@@ -11,6 +11,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -156,12 +157,46 @@ def test_null_handles_rejected(corr):
     assert ctypes.sizeof(corr.corrx_batch_item) == 72
 
 
+def test_packed_context_rejections(corr):
+    """corrx_create_packed checks the configuration as corr_create does, then the
+    sample width, before any device is touched."""
+    lib = corr.load()
+    h = ctypes.c_void_p(0x1)
+    cfg = corr.make_config(4, 2048, 64, 64)
+    for bits in (0, 1, 5, 16, -4):
+        assert lib.corrx_create_packed(ctypes.byref(cfg), bits, ctypes.byref(h)) == corr.ERR_ARG
+        assert not h.value and "sample_bits" in corr.last_error()
+    assert lib.corrx_create_packed(ctypes.byref(cfg), 4, None) == corr.ERR_ARG
+    bad = corr.make_config(4, 1000, 64, 64)  # the packet rule, as corr_create
+    assert lib.corrx_create_packed(ctypes.byref(bad), 4, ctypes.byref(h)) == corr.ERR_ARG
+
+
+def test_pack4_layout(corr):
+    """pack4 puts sample 2m in bits 0-3 and 2m+1 in bits 4-7 of byte m, two's
+    complement; decoding the bytes by sign-extending each nibble gives x back."""
+    x = np.array([[0, 1, -1, 7, -8, 3, 5, -6]], dtype=np.int8)
+    p = corr.pack4(x).view(np.uint8)
+    assert p.tolist() == [[0x10, 0x7F, 0x38, 0xA5]]
+    rng = np.random.default_rng(4)
+    y = rng.integers(-8, 8, size=(3, 5, 64)).astype(np.int8)
+    b = corr.pack4(y).astype(np.int16)
+    lo = ((b & 0xF) ^ 8) - 8
+    hi = b >> 4
+    back = np.empty_like(y)
+    back[..., 0::2], back[..., 1::2] = lo, hi
+    assert np.array_equal(back, y)
+    with pytest.raises(ValueError):
+        corr.pack4(np.array([8, 0], dtype=np.int8))
+
+
 @pytest.mark.parametrize("shape,status", [
     ((4, 1024, 16, 7), "ERR_ARG"),        # unknown form
     ((4, 1024, 16, -1), "ERR_ARG"),
     ((49, 1024, 16, 1), "ERR_NOT_IMPL"),  # FP16 form: at most 48 inputs
     ((49, 1024, 16, 2), "ERR_NOT_IMPL"),
     ((4, 1000, 16, 1), "ERR_NOT_IMPL"),   # FFT length
+    ((4, 3072, 16, 0), "ERR_NOT_IMPL"),   # odd factor, not a multiple of 2048
+    ((4, 18432, 16, 2), "ERR_NOT_IMPL"),  # 9 * 2048, above 16384
     ((0, 1024, 16, 2), "ERR_ARG"),
     ((4, 1024, 0, 1), "ERR_ARG"),
 ])

@@ -38,7 +38,7 @@ def test_device_dependent_rejections(corr):
         (dict(devices=[0, 64]), corr.ERR_ARG),
         (dict(devices=[0, 0]), corr.ERR_ARG),                  # a card driven twice
         (dict(n_station=45), corr.ERR_ARG),                    # 1035 > 1024 baselines
-        (dict(n_freq=6144), corr.ERR_NOT_IMPL),                # 3*2048: not a power of two
+        (dict(n_freq=18432), corr.ERR_NOT_IMPL),               # 9*2048: above 16384
         (dict(n_freq=32768), corr.ERR_NOT_IMPL),
     ]:
         cfg = small_cfg(corr, **kw)
@@ -49,10 +49,13 @@ def test_device_dependent_rejections(corr):
         assert corr.last_error()
 
 
-def test_staging_layout_and_mask_lifecycle(corr):
+@pytest.mark.parametrize("sample_bits", [8, 4])
+def test_staging_layout_and_mask_lifecycle(corr, sample_bits):
+    """abi_smoke.c:157-226 strides; with 4-bit staging (corrx_create_packed) a
+    frame takes n_freq bytes and everything else keeps its meaning."""
     cfg = small_cfg(corr)
-    with corr.Correlator(cfg) as c:
-        frame_stride = 2 * cfg.n_freq
+    with corr.Correlator(cfg, sample_bits=sample_bits) as c:
+        frame_stride = 2 * cfg.n_freq * sample_bits // 8
         slot_stride = frame_stride * cfg.frames_per_slot
         samples = slot_stride * cfg.block_slots
         mps = (cfg.frames_per_slot + 7) // 8
@@ -289,8 +292,11 @@ def test_rows_wait_and_fetch_submit(corr):
         ref2 = np.empty_like(ref)
         assert c.rows_fetch(ref2)[0] == 1
         # one device: laps and paths are bit-identical
-        assert ref.tobytes() == ref2.tobytes()
-        assert out.tobytes() == ref2.tobytes()
+        # compared as bools first: a failing comparison of large bytes objects
+        # would make pytest build a difflib diff of them
+        same_ref = ref.tobytes() == ref2.tobytes()
+        same_out = out.tobytes() == ref2.tobytes()
+        assert same_ref and same_out
         # argument checks (async_notify.c:457-502)
         assert lib.corr_rows_fetch_submit(c.h, out.ctypes.data, 8, ctypes.byref(rows), cnt.fn,
                                           None) == corr.ERR_ARG

@@ -34,7 +34,8 @@ def _spectra(hip, eng, N, T, nf):
     return out[:, : T * nf].reshape(N, T, nf)
 
 
-@pytest.mark.parametrize("nf", [512, 1024, 2048, 4096, 8192, 16384])
+# powers of two, and the multiples of 2048 with an odd factor (3, 5, 3, 7)
+@pytest.mark.parametrize("nf", [512, 1024, 2048, 4096, 8192, 16384, 6144, 10240, 12288, 14336])
 def test_fengine_vs_fp64(corr, oracle, hip, nf):
     N, T = 3, 4
     L = 2 * nf

@@ -79,11 +79,12 @@ def _x_int64(q, N):
 INT8_CASES = [(1, 512, 3, 0, 0), (4, 512, 70, 0, 0), (9, 1024, 130, 0, 0), (24, 512, 64, 0, 0),
               (24, 16384, 37, 0, 0), (30, 512, 20, 0, 0), (40, 512, 65, 0, 0), (48, 512, 7, 0, 0),
               (1, 512, 3, 32, 32), (5, 1024, 70, 32, -1), (16, 512, 64, 32, 32), (24, 16384, 37, 32, 32),
-              (24, 16384, 37, 16, -1), (24, 512, 130, 16, 16), (40, 1024, 65, 16, -1)]
+              (24, 16384, 37, 16, -1), (24, 512, 130, 16, 16), (40, 1024, 65, 16, -1),
+              (8, 6144, 33, 32, -1), (24, 14336, 20, 16, 16), (40, 10240, 9, 16, -1)]
 FP16_CASES = [(1, 512, 1, 0, 0), (5, 512, 33, 0, 0), (8, 1024, 64, 0, 0), (24, 512, 96, 0, 0),
               (24, 16384, 40, 0, 0), (40, 512, 31, 0, 0), (48, 512, 50, 0, 0),
               (3, 512, 33, 32, 32), (24, 16384, 40, 32, 32), (24, 16384, 40, 32, -1), (24, 512, 96, 16, 16),
-              (48, 512, 50, 16, -1)]
+              (48, 512, 50, 16, -1), (8, 6144, 33, 32, 32), (24, 12288, 20, 32, -1), (40, 14336, 9, 16, 16)]
 
 
 def _engine(corr, N, nf, T, form, ch, tile):
@@ -202,9 +203,9 @@ def _pipeline(corr, oracle, hip, form, N, nf, T, chunk, cal_frames, rho, seed, m
     return x, valid, gains, Yq, V, raw
 
 
-@pytest.mark.parametrize("form", [1, 2])
-def test_form_pipeline(corr, oracle, hip, form):
-    N, nf, T, chunk = 6, 2048, 100, 32
+@pytest.mark.parametrize("form,nf", [(1, 2048), (2, 2048), (1, 6144), (2, 10240), (1, 14336), (2, 12288)])
+def test_form_pipeline(corr, oracle, hip, form, nf):
+    N, T, chunk = 6, 100, 32
     print(f"form {form} pipeline N={N} nf={nf} T={T} chunk={chunk}", flush=True)
     x, valid, gains, Yq, V, raw = _pipeline(corr, oracle, hip, form, N, nf, T, chunk, 60, 0.3, 5, 8)
     assert np.all(Yq[1, 3] == 0)
@@ -280,7 +281,7 @@ def test_c3_shape_fp16_error(corr, oracle, hip):
     assert e32 < 1e-5
 
 
-@pytest.mark.parametrize("form,nf", [(1, 2048), (1, 16384), (2, 2048), (2, 16384)])
+@pytest.mark.parametrize("form,nf", [(1, 2048), (1, 16384), (2, 2048), (2, 16384), (1, 12288), (2, 14336)])
 def test_fengine_form_rounding(corr, oracle, hip, form, nf):
     """The form's F-engine output is the FP32 form's spectrum y requantized on
     the host: FP16 = y*g rounded to nearest even (numpy float16), int8 =

@@ -34,6 +34,8 @@ def golden_input(rows, N, n_cum, L, seed=20260914):
 @pytest.mark.parametrize("N,nf,n_cum,fps,rows", [
     (4, 2048, 300, 128, 2),   # golden_partial_slot
     (15, 4096, 200, 64, 2),   # golden_15st
+    (4, 6144, 150, 64, 2),    # n_freq with an odd factor: 3 x 2048
+    (3, 14336, 70, 32, 1),    # 7 x 2048
 ])
 def test_golden(corr, oracle, N, nf, n_cum, fps, rows):
     print(f"golden N={N} nf={nf} n_cum={n_cum} fps={fps} rows={rows}", flush=True)
@@ -62,7 +64,8 @@ def test_golden(corr, oracle, N, nf, n_cum, fps, rows):
 
     zeroed = drive(corr, cfg, x, rows, n_cum, HOST_ZEROED)
     masked = drive(corr, cfg, x, rows, n_cum, MASK_MARKED)
-    assert zeroed.tobytes() == masked.tobytes(), "device zeroing differs from host zeroing"
+    same = zeroed.tobytes() == masked.tobytes()  # a bool: no difflib diff of large bytes on failure
+    assert same, "device zeroing differs from host zeroing"
     assert rel_rms(zeroed, got) > 1e-6, "declaring frames missing changed nothing"
 
     valid = valid_matrix(rows, N, n_cum)
@@ -92,7 +95,8 @@ def test_whole_missing_chunks_skipped(corr, oracle):
     zs, ms = [], []
     zeroed = drive(corr, cfg, x, rows, n_cum, HOST_ZEROED, missing=_skip_pattern, stats=zs)
     masked = drive(corr, cfg, x, rows, n_cum, MASK_MARKED, missing=_skip_pattern, stats=ms)
-    assert zeroed.tobytes() == masked.tobytes(), "skipping differs from transforming zeros"
+    same = zeroed.tobytes() == masked.tobytes()
+    assert same, "skipping differs from transforming zeros"
     valid = valid_matrix(rows, N, n_cum, _skip_pattern)
     ref = np.stack([oracle.fx_f64(x[r], frames_div=n_cum, valid=valid[r]) for r in range(rows)])
     assert rel_rms(masked, ref) < TOL
@@ -101,3 +105,42 @@ def test_whole_missing_chunks_skipped(corr, oracle):
     print(f"  copied bytes: host-zeroed {zs[0].bytes_h2d}, mask-marked {ms[0].bytes_h2d}", flush=True)
     assert zs[0].bytes_h2d - ms[0].bytes_h2d == 5 * per_chunk
     assert zs[0].transfers - ms[0].transfers == 5
+
+
+@pytest.mark.parametrize("N,nf,n_cum,fps,chunk", [
+    (4, 2048, 150, 64, 32),
+    (3, 6144, 70, 32, 32),    # the mixed-radix transform
+    (5, 16384, 40, 16, 16),   # the half-buffer transform
+])
+def test_packed_staging(corr, oracle, N, nf, n_cum, fps, chunk):
+    """4-bit staging (corrx_create_packed, the reference's proposed next step,
+    HANDOFF.md:1437-1439): samples in [-8, 7] give rows byte-identical to the
+    8-bit context on the same samples (each chunk is unpacked to int8 on the
+    GPU and transformed by the same F-engine), with exactly half the sample
+    bytes copied; missing frames behave as in the 8-bit path."""
+    rows = 2
+    L = 2 * nf
+    print(f"packed staging N={N} nf={nf} n_cum={n_cum} fps={fps} chunk={chunk}", flush=True)
+    rng = np.random.default_rng(nf + N)
+    x = rng.integers(-8, 8, size=(rows, N, n_cum, L)).astype(np.int8)
+    x[0, 0, 0, :4] = [-8, 7, 7, -8]
+    cfg = corr.make_config(N, nf, fps, chunk, block_slots=3, rows_per_fetch=rows)
+    s8, s4 = [], []
+    got8 = drive(corr, cfg, x, rows, n_cum, ALL_VALID, stats=s8)
+    got4 = drive(corr, cfg, x, rows, n_cum, ALL_VALID, stats=s4, sample_bits=4)
+    same = got4.tobytes() == got8.tobytes()
+    assert same, f"4-bit staging changed the rows (max |difference| {np.max(np.abs(got4 - got8)):.3e})"
+    ref = np.stack([oracle.fx_f64(x[r], frames_div=n_cum) for r in range(rows)])
+    err = rel_rms(got4, ref)
+    print(f"  rel RMS vs FP64 oracle = {err:.3e}", flush=True)
+    assert err < TOL
+    mask_bytes = s8[0].transfers * N * ((fps + 7) // 8)
+    assert s4[0].transfers == s8[0].transfers
+    assert 2 * (s4[0].bytes_h2d - mask_bytes) == s8[0].bytes_h2d - mask_bytes
+    masked4 = drive(corr, cfg, x, rows, n_cum, MASK_MARKED, sample_bits=4)
+    masked8 = drive(corr, cfg, x, rows, n_cum, MASK_MARKED)
+    same = masked4.tobytes() == masked8.tobytes()
+    assert same, "4-bit staging changed the mask-marked rows"
+    valid = valid_matrix(rows, N, n_cum)
+    ref_m = np.stack([oracle.fx_f64(x[r], frames_div=n_cum, valid=valid[r]) for r in range(rows)])
+    assert rel_rms(masked4, ref_m) < TOL

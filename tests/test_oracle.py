@@ -19,7 +19,9 @@ from conftest import rel_rms
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
-@pytest.mark.parametrize("L", [8, 16, 64, 128])
+# powers of two, and odd parts 3, 5, 7, 9, 15 (n_freq = 2048 * odd is a valid
+# reference configuration)
+@pytest.mark.parametrize("L", [8, 16, 64, 128, 6, 24, 30, 40, 56, 72, 120, 3 * 64])
 def test_fft_matches_bruteforce_dft(oracle, L):
     rng = np.random.default_rng(L)
     x = rng.integers(-128, 128, L).astype(np.float64)
@@ -28,7 +30,8 @@ def test_fft_matches_bruteforce_dft(oracle, L):
     assert np.abs(a - b).max() <= 1e-11 * max(1.0, np.abs(b).max())
 
 
-@pytest.mark.parametrize("L", [2048, 4096, 8192, 16384, 32768])
+@pytest.mark.parametrize("L", [2048, 4096, 8192, 16384, 32768, 2 * 6144, 2 * 10240, 2 * 12288, 2 * 14336,
+                               2 * 18432])
 def test_fft_matches_numpy(oracle, L):
     rng = np.random.default_rng(L)
     x = rng.integers(-128, 128, L).astype(np.float64)
