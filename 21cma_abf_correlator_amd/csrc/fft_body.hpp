@@ -290,6 +290,27 @@ CORR_HD CORR_FI void unpack4(size_t i, int st, const uint32_t *in, size_t in_pit
     if (i < words) out[(size_t)st * out_pitch + i] = unpack4_word(in[(size_t)st * in_pitch + i]);
 }
 
+// An int8 pair as two floats, exactly. On the device each half is one
+// conversion that selects and sign-extends its byte of the 16-bit pair (SDWA
+// BYTE_0 / BYTE_1); the compiler otherwise extracts and extends each byte
+// with a shift or bit-field instruction first, two instructions per sample.
+// ALU operations only: the pair is already in a register.
+CORR_HD CORR_FI float2 int8_pair_to_float2(char2 c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const unsigned w = (unsigned)(uint8_t)c.x | ((unsigned)(uint8_t)c.y << 8);
+    float lo, hi;
+    asm("v_cvt_f32_i32_sdwa %0, sext(%1) dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:BYTE_0"
+        : "=v"(lo)
+        : "v"(w));
+    asm("v_cvt_f32_i32_sdwa %0, sext(%1) dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:BYTE_1"
+        : "=v"(hi)
+        : "v"(w));
+    return make_float2(lo, hi);
+#else
+    return make_float2((float)c.x, (float)c.y);
+#endif
+}
+
 // Thread j's 16 input values of frame f of station st, z[j + r*N/16] for
 // r < 16, converted exactly from int8 pairs.
 template <int LOG2N>
@@ -300,7 +321,7 @@ CORR_HD CORR_FI void load_frame(int j, const int8_t *in, size_t station_pitch, i
 #pragma unroll
     for (int r = 0; r < 16; ++r) raw[r] = src[(unsigned)(j + r * NT)];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) v[r] = make_float2((float)raw[r].x, (float)raw[r].y);
+    for (int r = 0; r < 16; ++r) v[r] = int8_pair_to_float2(raw[r]);
 }
 
 // Frame f of station st is present: `valid` is NULL (every frame present) or
