@@ -4,6 +4,7 @@
 #ifndef CORR_HD
 #define CORR_HD __host__ __device__
 #endif
+#include "fft_body.hpp"
 #include "xengine_body.hpp"
 #include "xmfma_body.hpp"
 
@@ -45,15 +46,22 @@ corr_status hip_check(hipError_t e, const char *what) {
 }
 
 bool fft_size_supported(int n_freq) {
-    if (n_freq < 512 || n_freq > 16384) return false;
+    if (n_freq > 16384) return fft::staged_plan(n_freq).m > 0;
+    if (n_freq < 512) return false;
     if ((n_freq & (n_freq - 1)) == 0) return true;
     return n_freq == 6144 || n_freq == 10240 || n_freq == 12288 || n_freq == 14336;
 }
 
 int fft_twiddle_elems(int n_freq) {
-    const int p = n_freq & -n_freq;  // largest power of two dividing n_freq
+    int p = n_freq & -n_freq;  // largest power of two dividing n_freq
+    if (p > 16384) p = 16384;
     return p == n_freq ? n_freq : n_freq + p;
 }
+
+// Frames per batch of the staged transform: its two buffers then take
+// 2 * 64 * 8 bytes per input sample pair of a frame (at 24 inputs and n_freq
+// 65536, 1.6 GB).
+constexpr int kStageFrames = 64;
 
 corr_status Engine::create(const EngineShape &shape, Engine **out) {
     if (out == nullptr) return fail(CORR_ERR_ARG, "engine: out is NULL");
@@ -62,8 +70,8 @@ corr_status Engine::create(const EngineShape &shape, Engine **out) {
     if (shape.chunk_frames < 1) return fail(CORR_ERR_ARG, "engine: chunk_frames must be >= 1");
     if (!fft_size_supported(shape.n_freq))
         return failf(CORR_ERR_NOT_IMPL,
-                     "n_freq=%d: this build transforms n_freq = 2^k in [512, 16384] and "
-                     "6144, 10240, 12288, 14336 only (FFT length 2*n_freq)",
+                     "n_freq=%d: this build transforms n_freq = 2^k in [512, 16384], 6144, 10240, "
+                     "12288, 14336, and the multiples of 2048 up to 65536 (FFT length 2*n_freq)",
                      shape.n_freq);
     if (shape.form != Form::kFP32 && shape.form != Form::kFP16 && shape.form != Form::kINT8)
         return failf(CORR_ERR_ARG, "engine: unknown form %d", (int)shape.form);
@@ -103,6 +111,15 @@ corr_status Engine::init() {
     const size_t spec_bytes = (size_t)N * (size_t)nf * per_station;
     if ((st = hip_check(hipMalloc(&d_spec_, spec_bytes), "hipMalloc(spectra)")) != CORR_OK)
         return st;
+    if (fft::staged_plan(nf).m > 0) {
+        stage_frames_ = shape_.chunk_frames < kStageFrames ? shape_.chunk_frames : kStageFrames;
+        const size_t stage = sizeof(float2) * (size_t)N * (size_t)stage_frames_ * (size_t)nf;
+        if ((st = hip_check(hipMalloc(reinterpret_cast<void **>(&d_stage_y_), stage), "hipMalloc(staged transform)")) !=
+                CORR_OK ||
+            (st = hip_check(hipMalloc(reinterpret_cast<void **>(&d_stage_z_), stage), "hipMalloc(staged transform)")) !=
+                CORR_OK)
+            return st;
+    }
     if ((st = hip_check(hipMalloc(reinterpret_cast<void **>(&d_valid_),
                                   sizeof(int32_t) * (size_t)N * (size_t)shape_.chunk_frames),
                         "hipMalloc(validity)")) != CORR_OK)
@@ -244,10 +261,8 @@ corr_status Engine::tune_xtile() {
         (st = hip_check(launch_fill(in, in_bytes / 4, 0u, s), "fill_kernel(input)")) == CORR_OK &&
         (st = hip_check(launch_fill(integ, integ_bytes / 4, 0u, s), "fill_kernel(integrator)")) == CORR_OK) {
         auto round = [&](const std::pair<int, int> &c) {
-            corr_status r = hip_check(launch_fengine_form((int)shape_.form, nf, static_cast<const int8_t *>(in),
-                                                          0, nullptr, 0, d_tw_, d_wl_, d_gain_, d_spec_,
-                                                          spectra_station_stride(), c.second, frames, N, s),
-                                      "fengine_form_kernel");
+            corr_status r = transform((int)shape_.form, static_cast<const int8_t *>(in), 0, nullptr, 0, frames,
+                                      d_spec_, spectra_station_stride(), c.second, s);
             if (r != CORR_OK) return r;
             return hip_check(launch_xengine_mfma((int)shape_.form, N, c.first, d_spec_, spectra_station_stride(),
                                                  nf, c.second, frames, integ, s),
@@ -309,6 +324,8 @@ Engine::~Engine() {
         if (ev_x_[i] != nullptr) hipEventDestroy(ev_x_[i]);
     }
     hipFree(d_spec_b_);
+    hipFree(d_stage_y_);
+    hipFree(d_stage_z_);
     hipFree(d_tw_);
     hipFree(d_wl_);
     hipFree(d_spec_);
@@ -441,14 +458,31 @@ corr_status Engine::fengine_to(void *spec, const int8_t *d_in, size_t station_pi
         if (st != CORR_OK) return st;
         valid = d_valid_;
     }
-    if (shape_.form == Form::kFP32)
-        return hip_check(launch_fengine_fp32(shape_.n_freq, d_in, station_pitch, valid, valid_stride, d_tw_,
-                                             d_wl_, static_cast<float2 *>(spec),
-                                             spectra_station_stride(), frames, shape_.n_station, s),
+    return transform((int)shape_.form, d_in, station_pitch, valid, valid_stride, frames, spec,
+                     spectra_station_stride(), spec_tile_, s);
+}
+
+corr_status Engine::transform(int form, const int8_t *d_in, size_t station_pitch, const int32_t *valid,
+                              size_t valid_stride, int frames, void *out, size_t out_station_stride, int tile,
+                              hipStream_t s) {
+    const int N = shape_.n_station, nf = shape_.n_freq;
+    if (d_stage_y_ != nullptr) {
+        for (int f0 = 0; f0 < frames; f0 += stage_frames_) {
+            const int n = frames - f0 < stage_frames_ ? frames - f0 : stage_frames_;
+            corr_status st = hip_check(launch_fengine_staged(form, nf, d_in, station_pitch, valid, valid_stride, f0, n,
+                                                             N, d_tw_, d_wl_, d_gain_, out, out_station_stride, tile,
+                                                             d_stage_y_, d_stage_z_, s),
+                                       "staged transform kernels");
+            if (st != CORR_OK) return st;
+        }
+        return CORR_OK;
+    }
+    if (form == 0)
+        return hip_check(launch_fengine_fp32(nf, d_in, station_pitch, valid, valid_stride, d_tw_, d_wl_,
+                                             static_cast<float2 *>(out), out_station_stride, frames, N, s),
                          "fengine_fp32_kernel");
-    return hip_check(launch_fengine_form((int)shape_.form, shape_.n_freq, d_in, station_pitch, valid,
-                                         valid_stride, d_tw_, d_wl_, d_gain_, spec,
-                                         spectra_station_stride(), spec_tile_, frames, shape_.n_station, s),
+    return hip_check(launch_fengine_form(form, nf, d_in, station_pitch, valid, valid_stride, d_tw_, d_wl_, d_gain_,
+                                         out, out_station_stride, tile, frames, N, s),
                      "fengine_form_kernel");
 }
 
@@ -504,9 +538,8 @@ corr_status Engine::calibrate(const int8_t *d_in, size_t station_pitch, const ui
                 return st;
             valid = d_valid_;
         }
-        if ((st = hip_check(launch_fengine_fp32(nf, d_in + (size_t)b0 * 2u * (size_t)nf, station_pitch,
-                                                valid, valid_stride, d_tw_, d_wl_, spec, cal_stride, nb, N, s),
-                            "fengine_fp32_kernel")) != CORR_OK ||
+        if ((st = transform(0, d_in + (size_t)b0 * 2u * (size_t)nf, station_pitch, valid, valid_stride, nb, spec,
+                            cal_stride, nf, s)) != CORR_OK ||
             (st = hip_check(launch_power(nf, nb, N, spec, cal_stride, valid, valid_stride, d_power_, d_count_, s),
                             "power_kernel")) != CORR_OK)
             return st;

@@ -46,14 +46,16 @@ constexpr int kMaxBaselinesAbi = 1024;
 /// Largest input count of the FP16/int8 forms (2*n_station <= 96 Gram rows).
 constexpr int kMaxStationMfma = 48;
 
-/// FFT sizes this build transforms: n_freq = 2^k with 512 <= n_freq <= 16384,
-/// and the multiples of 2048 below 16384 with an odd factor, 6144 (3 x 2048),
-/// 10240 (5 x 2048), 12288 (3 x 4096) and 14336 (7 x 2048).
+/// FFT sizes this build transforms: n_freq = 2^k with 512 <= n_freq <= 16384;
+/// the multiples of 2048 below 16384 with an odd factor, 6144 (3 x 2048),
+/// 10240 (5 x 2048), 12288 (3 x 4096) and 14336 (7 x 2048), in one workgroup's
+/// LDS; and every multiple of 2048 from 18432 to 65536, staged through HBM.
 bool fft_size_supported(int n_freq);
 
 /// Twiddle table entries of a supported n_freq: exp(-2 pi i m/n_freq) for
-/// m < n_freq, then, for n_freq = M * P with an odd M > 1, exp(-2 pi i m/P)
-/// for m < P (the power-of-two sub-transforms).
+/// m < n_freq, then, for n_freq = M * P with M > 1 (P the largest power of two
+/// dividing n_freq, at most 16384), exp(-2 pi i m/P) for m < P (the
+/// power-of-two sub-transforms).
 int fft_twiddle_elems(int n_freq);
 
 struct EngineShape {
@@ -154,6 +156,13 @@ class Engine {
 
     corr_status fengine_to(void *spec, const int8_t *d_in, size_t station_pitch, const uint8_t *d_mask,
                            size_t mask_pitch, int mask_bit0, int frames, hipStream_t s);
+    /// The transform of `frames` frames (validity already expanded, or NULL)
+    /// into `out` in representation `form` (0 FP32, 1 FP16, 2 int8; tile of the
+    /// spectrum order for 1 and 2): one kernel, or for n_freq above 16384 the
+    /// staged kernels in batches of stage_frames_.
+    corr_status transform(int form, const int8_t *d_in, size_t station_pitch, const int32_t *valid,
+                          size_t valid_stride, int frames, void *out, size_t out_station_stride, int tile,
+                          hipStream_t s);
     corr_status xengine_from(const void *spec, void *integ, int frames, hipStream_t s);
     corr_status ensure_pipeline();
     bool overlaps(int64_t frames) const { return shape_.form != Form::kFP32 && frames > shape_.chunk_frames; }
@@ -176,6 +185,10 @@ class Engine {
     float2 *d_wl_ = nullptr;    ///< exp(-2 pi i k / (2 n_freq)), k < n_freq
     void *d_spec_ = nullptr;
     int32_t *d_valid_ = nullptr;  ///< expanded validity, [station][chunk_frames]
+    /// n_freq above 16384: the staged transform's buffers, [station][frame][n_freq]
+    /// complex64 for stage_frames_ frames each
+    float2 *d_stage_y_ = nullptr, *d_stage_z_ = nullptr;
+    int stage_frames_ = 0;
     int2 *d_pairs_ = nullptr;     ///< (j, k) of every baseline
     float *d_gain_ = nullptr;     ///< FP16/int8 forms: [station][n_freq]
     float *d_power_ = nullptr;    ///< calibration: [station][n_freq] sum of |Y|^2
@@ -193,6 +206,13 @@ class Engine {
 // Kernel launchers (engine_kernels.hip).
 /// 4-bit packed frames (n_freq bytes each, in_pitch bytes per station) to int8
 /// frames (2*n_freq bytes, out_pitch per station), `frames` frames of each station.
+/// n_freq above 16384: stages 1-3 of the staged transform for `frames` frames
+/// starting at frame0 (input, validity and output are indexed by frame0 + f;
+/// ys, zs hold [station][frames][n_freq] complex64).
+hipError_t launch_fengine_staged(int form, int n_freq, const int8_t *in, size_t station_pitch, const int32_t *valid,
+                                 size_t valid_stride, int frame0, int frames, int n_station, const float2 *tw,
+                                 const float2 *wl, const float *gain, void *out, size_t out_station_stride, int tile,
+                                 float2 *ys, float2 *zs, hipStream_t s);
 hipError_t launch_unpack4(const int8_t *in, size_t in_pitch, int8_t *out, size_t out_pitch, int frames,
                           int n_freq, int n_station, hipStream_t s);
 /// tw: fft_twiddle_elems(n_freq) entries; wl: n_freq.

@@ -131,6 +131,70 @@ fengine_mixed_form_kernel(const int8_t *__restrict__ in, size_t station_pitch,
                                                   out_station_stride, tile);
 }
 
+// n_freq above 16384: the transform staged through HBM (fft_body.hpp
+// staged_*). Stage 1: grid (ceil(P/256), m, frames * stations).
+constexpr unsigned kStagedBlock = 256;
+__global__ void __launch_bounds__(kStagedBlock)
+staged_first_kernel(const int8_t *__restrict__ in, size_t station_pitch, const int32_t *__restrict__ valid,
+                    size_t valid_stride, int frame0, int frames, int n, int m, int p,
+                    const float2 *__restrict__ tw, float2 *__restrict__ ys) {
+    fft::StagedPlan pl;
+    pl.n = n;
+    pl.m = m;
+    pl.p = p;
+    const int f = (int)(blockIdx.z % (unsigned)frames), st = (int)(blockIdx.z / (unsigned)frames);
+    fft::staged_first((int)(blockIdx.x * kStagedBlock + threadIdx.x), (int)blockIdx.y, f, st, frames, pl, in,
+                      station_pitch, frame0, valid, valid_stride, tw, ys);
+}
+
+// Stage 2: grid (m, frames, stations), P/16 lanes, the P-point exchange buffer
+// (139 KB at P = 16384).
+template <int LOG2P>
+__global__ void __launch_bounds__((1 << LOG2P) / 16)
+staged_sub_kernel(const int32_t *__restrict__ valid, size_t valid_stride, int frame0, int frames, int m,
+                  const float2 *__restrict__ ys, const float2 *__restrict__ twp, float2 *__restrict__ zs) {
+    __shared__ float2 lds[fft::Sizes<(1 << LOG2P)>::kLds];
+    DeviceSync sync;
+    fft::staged_sub<LOG2P>((int)threadIdx.x, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, frames, m, frame0,
+                           valid, valid_stride, lds, sync, ys, twp, zs);
+}
+
+// Stage 3: grid (ceil((N/2+1)/256), frames, stations). FORM 0: complex64
+// [station][frame][N]; FORM 1/2: the form's words in the tile-major order of
+// forms::spec_index (tile 16, 32 or N), bin k scaled by gain[st*N + k].
+template <int FORM>
+__global__ void __launch_bounds__(kStagedBlock)
+staged_r2c_kernel(const int32_t *__restrict__ valid, size_t valid_stride, int frame0, int frames, int n, int m,
+                  int p, const float2 *__restrict__ zs, const float2 *__restrict__ wl,
+                  const float *__restrict__ gain, void *__restrict__ out, size_t out_station_stride, int tile) {
+    fft::StagedPlan pl;
+    pl.n = n;
+    pl.m = m;
+    pl.p = p;
+    const int f = (int)blockIdx.y, st = (int)blockIdx.z;
+    const int frame = frame0 + f;
+    const int fcap = (int)(out_station_stride / (size_t)n);
+    auto at = [&](int k) { return forms::spec_index(st, frame, k, n, fcap, tile); };
+    if constexpr (FORM == 0) {
+        float2 *o = static_cast<float2 *>(out) + (size_t)st * out_station_stride + (size_t)frame * (size_t)n;
+        fft::staged_r2c((int)(blockIdx.x * kStagedBlock + threadIdx.x), f, st, frames, pl, frame0, valid,
+                        valid_stride, zs, wl, [&](int k, float2 x) { o[k] = x; },
+                        [&](int k) { o[k] = make_float2(0.0f, 0.0f); });
+    } else if constexpr (FORM == 1) {
+        uint32_t *o = static_cast<uint32_t *>(out);
+        const float *g = gain + (size_t)st * (size_t)n;
+        fft::staged_r2c((int)(blockIdx.x * kStagedBlock + threadIdx.x), f, st, frames, pl, frame0, valid,
+                        valid_stride, zs, wl, [&](int k, float2 x) { o[at(k)] = forms::pack_fp16(x, g[k]); },
+                        [&](int k) { o[at(k)] = 0u; });
+    } else {
+        uint16_t *o = static_cast<uint16_t *>(out);
+        const float *g = gain + (size_t)st * (size_t)n;
+        fft::staged_r2c((int)(blockIdx.x * kStagedBlock + threadIdx.x), f, st, frames, pl, frame0, valid,
+                        valid_stride, zs, wl, [&](int k, float2 x) { o[at(k)] = forms::pack_int8(x, g[k]); },
+                        [&](int k) { o[at(k)] = (uint16_t)0; });
+    }
+}
+
 // 4-bit packed samples -> int8, one 32-bit word (8 samples) per lane;
 // grid (ceil(words/256), stations).
 constexpr unsigned kUnpackBlock = 256;
@@ -436,6 +500,52 @@ hipError_t launch_fengine_fp32(int n_freq, const int8_t *in, size_t station_pitc
     default:
         return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_fengine_staged(int form, int n_freq, const int8_t *in, size_t station_pitch, const int32_t *valid,
+                                 size_t valid_stride, int frame0, int frames, int n_station, const float2 *tw,
+                                 const float2 *wl, const float *gain, void *out, size_t out_station_stride, int tile,
+                                 float2 *ys, float2 *zs, hipStream_t s) {
+    if (frames <= 0 || n_station <= 0) return hipSuccess;
+    const fft::StagedPlan pl = fft::staged_plan(n_freq);
+    if (pl.m == 0 || form < 0 || form > 2 || (form != 0 && tile != 16 && tile != 32 && tile != n_freq) ||
+        (size_t)frames * (size_t)n_station > 65535u)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(staged_first_kernel, dim3((unsigned)((pl.p + kStagedBlock - 1) / kStagedBlock), (unsigned)pl.m,
+                                                 (unsigned)(frames * n_station)),
+                       dim3(kStagedBlock), 0, s, in, station_pitch, valid, valid_stride, frame0, frames, pl.n, pl.m,
+                       pl.p, tw, ys);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const dim3 sub_grid((unsigned)pl.m, (unsigned)frames, (unsigned)n_station);
+    const dim3 sub_block((unsigned)(pl.p / 16));
+    switch (pl.log2p) {
+#define CORR_STAGED_CASE(L)                                                                                          \
+    case L:                                                                                                           \
+        hipLaunchKernelGGL(staged_sub_kernel<L>, sub_grid, sub_block, 0, s, valid, valid_stride, frame0, frames, pl.m, \
+                           ys, tw + n_freq, zs);                                                                      \
+        break;
+        CORR_STAGED_CASE(11)
+        CORR_STAGED_CASE(12)
+        CORR_STAGED_CASE(13)
+        CORR_STAGED_CASE(14)
+#undef CORR_STAGED_CASE
+    default:
+        return hipErrorInvalidValue;
+    }
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    const dim3 r2c_grid((unsigned)((n_freq / 2 + 1 + kStagedBlock - 1) / kStagedBlock), (unsigned)frames,
+                        (unsigned)n_station);
+    if (form == 0)
+        hipLaunchKernelGGL(staged_r2c_kernel<0>, r2c_grid, dim3(kStagedBlock), 0, s, valid, valid_stride, frame0,
+                           frames, pl.n, pl.m, pl.p, zs, wl, gain, out, out_station_stride, tile);
+    else if (form == 1)
+        hipLaunchKernelGGL(staged_r2c_kernel<1>, r2c_grid, dim3(kStagedBlock), 0, s, valid, valid_stride, frame0,
+                           frames, pl.n, pl.m, pl.p, zs, wl, gain, out, out_station_stride, tile);
+    else
+        hipLaunchKernelGGL(staged_r2c_kernel<2>, r2c_grid, dim3(kStagedBlock), 0, s, valid, valid_stride, frame0,
+                           frames, pl.n, pl.m, pl.p, zs, wl, gain, out, out_station_stride, tile);
+    return hipGetLastError();
 }
 
 hipError_t launch_unpack4(const int8_t *in, size_t in_pitch, int8_t *out, size_t out_pitch, int frames,

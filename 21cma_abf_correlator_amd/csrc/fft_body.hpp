@@ -967,6 +967,101 @@ CORR_HD CORR_FI void fengine_mixed_form_block(int j, int f, int st, float2 *lds,
     fengine_mixed_frame<M, LOG2P>(j, f, st, lds, sync, in, station_pitch, valid, valid_stride, tw, wl, e);
 }
 
+// ---------------------------------------------------------------------------
+// n_freq above 16384 (every multiple of 2048 up to 65536, which the reference
+// accepts, corr-cuda/src/corr_cuda.cpp:500-509): one frame's transform no
+// longer fits a workgroup's LDS, so it is staged through HBM in three kernels,
+// N = m P with P the largest power of two dividing N, at most 16384, and m <= 32:
+//   1. radix-m stage from the int8 input, one lane per (n1, k2):
+//        y_k2[n1] = W_N^(n1 k2) sum_n2 z[n1 + P n2] W_m^(n2 k2)
+//      with W_m^r = W_N^(r P), the phase n2 k2 reduced mod m exactly;
+//   2. the m P-point transforms, one workgroup each, the Stockham passes
+//      above in LDS: Z[m k1 + k2] = DFT_P(y_k2)[k1], stored as Zs[k2][k1];
+//   3. the R2C unpack of bins k and N-k from Z, one lane per pair, written in
+//      the form's representation and spectrum order.
+// The staging buffers hold [station][frame][m][P] complex64 of one batch of
+// frames. A missing frame skips stages 1 and 2 and is written as zeros by 3.
+// ---------------------------------------------------------------------------
+struct StagedPlan {
+    int n = 0, m = 0, p = 0, log2p = 0;
+};
+
+// The plan of an n_freq above 16384, or m = 0 where there is none.
+CORR_HD inline StagedPlan staged_plan(int n_freq) {
+    StagedPlan s;
+    if (n_freq <= 16384 || n_freq > 65536 || n_freq % 2048 != 0) return s;
+    int p = n_freq & -n_freq;
+    if (p > 16384) p = 16384;
+    s.n = n_freq;
+    s.p = p;
+    s.m = n_freq / p;
+    while ((1 << s.log2p) < p) ++s.log2p;
+    return s;
+}
+
+// Stage 1, lane (n1, k2) of frame f of station st; ys = the batch's staging
+// buffer, frame f at ((st * frames + f) * m + k2) * P.
+CORR_HD CORR_FI void staged_first(int n1, int k2, int f, int st, int frames, const StagedPlan &pl,
+                                 const int8_t *in, size_t station_pitch, int frame0, const int32_t *valid,
+                                 size_t valid_stride, const float2 *twn, float2 *ys) {
+    if (n1 >= pl.p || !frame_present(valid, valid_stride, frame0 + f, st)) return;
+    const char2 *z = reinterpret_cast<const char2 *>(in + (size_t)st * station_pitch +
+                                                     (size_t)(frame0 + f) * (size_t)(2 * pl.n));
+    float2 acc = make_float2(0.0f, 0.0f);
+    int r = 0;  // n2 * k2 mod m
+    for (int n2 = 0; n2 < pl.m; ++n2) {
+        const char2 c = z[(unsigned)(n1 + n2 * pl.p)];
+        const float2 w = twn[(unsigned)(r * pl.p)];
+        acc.x += (float)c.x * w.x - (float)c.y * w.y;
+        acc.y += (float)c.x * w.y + (float)c.y * w.x;
+        r += k2;
+        if (r >= pl.m) r -= pl.m;
+    }
+    ys[(((size_t)st * frames + f) * pl.m + k2) * pl.p + n1] = cmul(acc, twn[(unsigned)(n1 * k2)]);
+}
+
+// Stage 2, lane j of the workgroup of sub-sequence k2 of frame f of station st
+// (P/16 lanes): the P-point transform of y_k2 into zs at the same place.
+template <int LOG2P, class Sync>
+CORR_HD CORR_FI void staged_sub(int j, int k2, int f, int st, int frames, int m, int frame0, const int32_t *valid,
+                               size_t valid_stride, float2 *lds, Sync &sync, const float2 *ys, const float2 *twp,
+                               float2 *zs) {
+    constexpr int P = 1 << LOG2P, NT = P / 16;
+    if (!frame_present(valid, valid_stride, frame0 + f, st)) return;  // uniform over the workgroup
+    const size_t base = (((size_t)st * frames + f) * m + k2) * P;
+    float2 v[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = ys[base + (size_t)(j + r * NT)];
+    fft_from_values<LOG2P>(j, v, lds, sync, twp);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) zs[base + (size_t)(j + r * NT)] = lds[j + r * NT];
+}
+
+// Stage 3, lane k (0 <= k <= N/2) of frame f of station st: bins k and N-k
+// (k alone for 0 and N/2) from Z[k] = zs[k mod m][k div m]. `put(k, X)`
+// stores bin k, `zero(k)` a missing frame's bin.
+template <class Put, class Zero>
+CORR_HD CORR_FI void staged_r2c(int k, int f, int st, int frames, const StagedPlan &pl, int frame0,
+                               const int32_t *valid, size_t valid_stride, const float2 *zs, const float2 *wl,
+                               Put &&put, Zero &&zero) {
+    const int n = pl.n;
+    if (k > n / 2) return;
+    const bool pair = k != 0 && k != n / 2;
+    if (!frame_present(valid, valid_stride, frame0 + f, st)) {
+        zero(k);
+        if (pair) zero(n - k);
+        return;
+    }
+    const float2 *zf = zs + ((size_t)st * frames + f) * (size_t)n;
+    const int km = n - k == n ? 0 : n - k;
+    const float2 a = zf[(size_t)(k % pl.m) * pl.p + (size_t)(k / pl.m)];
+    const float2 b = zf[(size_t)(km % pl.m) * pl.p + (size_t)(km / pl.m)];
+    float2 xk, xm;
+    r2c_from(a, b, wl[(unsigned)k], xk, xm);
+    put(k, xk);
+    if (pair) put(n - k, xm);
+}
+
 // Calibration: power[st*nf + c] += sum over the frames present of |Y|^2, in
 // frame order; count[st] += frames present (by the channel-0 thread).
 // Thread c of the workgroup row of station st.

@@ -178,6 +178,69 @@ hipError_t launch_fengine_fp32(int n_freq, const int8_t *in, size_t station_pitc
     }
 }
 
+hipError_t launch_fengine_staged(int form, int n_freq, const int8_t *in, size_t station_pitch, const int32_t *valid,
+                                 size_t valid_stride, int frame0, int frames, int n_station, const float2 *tw,
+                                 const float2 *wl, const float *gain, void *out, size_t out_station_stride, int tile,
+                                 float2 *ys, float2 *zs, hipStream_t) {
+    if (frames <= 0 || n_station <= 0) return hipSuccess;
+    const fft::StagedPlan pl = fft::staged_plan(n_freq);
+    if (pl.m == 0 || form < 0 || form > 2 || (form != 0 && tile != 16 && tile != 32 && tile != n_freq) ||
+        (size_t)frames * (size_t)n_station > 65535u)
+        return hipErrorInvalidValue;
+    // stage 1, every lane of the device grid (ceil(P/256) x 256 lanes per (k2, frame, station))
+    const int lanes1 = (pl.p + 255) / 256 * 256;
+    for (int st = 0; st < n_station; ++st)
+        for (int f = 0; f < frames; ++f)
+            for (int k2 = 0; k2 < pl.m; ++k2)
+                for (int n1 = 0; n1 < lanes1; ++n1)
+                    fft::staged_first(n1, k2, f, st, frames, pl, in, station_pitch, frame0, valid, valid_stride, tw, ys);
+    // stage 2, one workgroup of P/16 lanes per (k2, frame, station)
+    auto sub = [&](auto log2p_tag) {
+        constexpr int L = decltype(log2p_tag)::value;
+        run_grid(pl.m, frames * n_station, (1 << L) / 16, fft::Sizes<(1 << L)>::kLds * sizeof(float2),
+                 [=](int j, int bx, int by, char *lds, HostSync &sync) {
+                     fft::staged_sub<L>(j, bx, by % frames, by / frames, frames, pl.m, frame0, valid, valid_stride,
+                                        reinterpret_cast<float2 *>(lds), sync, ys, tw + n_freq, zs);
+                 });
+    };
+    switch (pl.log2p) {
+    case 11: sub(std::integral_constant<int, 11>{}); break;
+    case 12: sub(std::integral_constant<int, 12>{}); break;
+    case 13: sub(std::integral_constant<int, 13>{}); break;
+    case 14: sub(std::integral_constant<int, 14>{}); break;
+    default: return hipErrorInvalidValue;
+    }
+    // stage 3, every lane of the device grid
+    const int lanes3 = (n_freq / 2 + 1 + 255) / 256 * 256;
+    const int fcap = (int)(out_station_stride / (size_t)n_freq);
+    for (int st = 0; st < n_station; ++st)
+        for (int f = 0; f < frames; ++f) {
+            const int frame = frame0 + f;
+            for (int k = 0; k < lanes3; ++k) {
+                if (form == 0) {
+                    float2 *o = static_cast<float2 *>(out) + (size_t)st * out_station_stride + (size_t)frame * n_freq;
+                    fft::staged_r2c(k, f, st, frames, pl, frame0, valid, valid_stride, zs, wl,
+                                    [&](int b, float2 x) { o[b] = x; }, [&](int b) { o[b] = make_float2(0.0f, 0.0f); });
+                } else {
+                    const float *g = gain + (size_t)st * n_freq;
+                    auto at = [&](int b) { return forms::spec_index(st, frame, b, n_freq, fcap, tile); };
+                    if (form == 1) {
+                        uint32_t *o = static_cast<uint32_t *>(out);
+                        fft::staged_r2c(k, f, st, frames, pl, frame0, valid, valid_stride, zs, wl,
+                                        [&](int b, float2 x) { o[at(b)] = forms::pack_fp16(x, g[b]); },
+                                        [&](int b) { o[at(b)] = 0u; });
+                    } else {
+                        uint16_t *o = static_cast<uint16_t *>(out);
+                        fft::staged_r2c(k, f, st, frames, pl, frame0, valid, valid_stride, zs, wl,
+                                        [&](int b, float2 x) { o[at(b)] = forms::pack_int8(x, g[b]); },
+                                        [&](int b) { o[at(b)] = (uint16_t)0; });
+                    }
+                }
+            }
+        }
+    return hipSuccess;
+}
+
 hipError_t launch_unpack4(const int8_t *in, size_t in_pitch, int8_t *out, size_t out_pitch, int frames,
                           int n_freq, int n_station, hipStream_t) {
     if (frames <= 0 || n_station <= 0) return hipSuccess;

@@ -349,11 +349,93 @@ static int check_unpack4() {
     return bad;
 }
 
+// n_freq above 16384: the three staged bodies in launch order, FP32 against
+// the oracle's FP64 FFT (any length), one missing frame written as zeros.
+static int check_staged(int n) {
+    const StagedPlan pl = staged_plan(n);
+    const int NST = 2, FR = 2;
+    const size_t pitch = (size_t)FR * 2 * n + 16;
+    std::mt19937 rng(7 + n);
+    std::uniform_int_distribution<int> d(-128, 127);
+    std::vector<int8_t> in(pitch * NST);
+    for (auto &v : in) v = (int8_t)d(rng);
+    std::vector<int32_t> valid = {1, 1, 1, 0};  // station 1, frame 1 missing
+    std::vector<float2> tw(n + pl.p), wl(n);
+    for (int m = 0; m < n; ++m) {
+        const double a = -2.0 * M_PI * m / n, b = -M_PI * m / n;
+        tw[m] = make_float2((float)std::cos(a), (float)std::sin(a));
+        wl[m] = make_float2((float)std::cos(b), (float)std::sin(b));
+    }
+    for (int m = 0; m < pl.p; ++m)
+        tw[n + m] = make_float2((float)std::cos(-2.0 * M_PI * m / pl.p), (float)std::sin(-2.0 * M_PI * m / pl.p));
+    std::vector<float2> ys((size_t)NST * FR * n), zs((size_t)NST * FR * n), out((size_t)NST * FR * n);
+    for (int st = 0; st < NST; ++st)
+        for (int f = 0; f < FR; ++f)
+            for (int k2 = 0; k2 < pl.m; ++k2)
+                for (int n1 = 0; n1 < (pl.p + 255) / 256 * 256; ++n1)
+                    staged_first(n1, k2, f, st, FR, pl, in.data(), pitch, 0, valid.data(), FR, tw.data(), ys.data());
+    auto sub = [&](auto tag) {
+        constexpr int L = decltype(tag)::value;
+        for (int st = 0; st < NST; ++st)
+            for (int f = 0; f < FR; ++f)
+                for (int k2 = 0; k2 < pl.m; ++k2) {
+                    float2 *lds = new float2[Sizes<(1 << L)>::kLds];
+                    std::barrier<> bar((1 << L) / 16);
+                    std::vector<std::thread> th;
+                    for (int j = 0; j < (1 << L) / 16; ++j)
+                        th.emplace_back([&, j, st, f, k2] {
+                            HostSync sync{&bar};
+                            staged_sub<L>(j, k2, f, st, FR, pl.m, 0, valid.data(), FR, lds, sync, ys.data(),
+                                          tw.data() + n, zs.data());
+                        });
+                    for (auto &t : th) t.join();
+                    delete[] lds;
+                }
+    };
+    if (pl.log2p == 11) sub(std::integral_constant<int, 11>{});
+    else if (pl.log2p == 12) sub(std::integral_constant<int, 12>{});
+    else if (pl.log2p == 13) sub(std::integral_constant<int, 13>{});
+    else sub(std::integral_constant<int, 14>{});
+    for (int st = 0; st < NST; ++st)
+        for (int f = 0; f < FR; ++f) {
+            float2 *o = out.data() + ((size_t)st * FR + f) * n;
+            for (int k = 0; k < (n / 2 + 1 + 255) / 256 * 256; ++k)
+                staged_r2c(k, f, st, FR, pl, 0, valid.data(), FR, zs.data(), wl.data(),
+                           [&](int b, float2 x) { o[b] = x; }, [&](int b) { o[b] = make_float2(0.0f, 0.0f); });
+        }
+    int bad = 0;
+    double worst = 0.0;
+    std::vector<double> x(2 * (size_t)n), ref(2 * (size_t)n);
+    for (int st = 0; st < NST; ++st)
+        for (int f = 0; f < FR; ++f) {
+            const float2 *y = out.data() + ((size_t)st * FR + f) * n;
+            if (!valid[(size_t)st * FR + f]) {
+                for (int k = 0; k < n; ++k)
+                    if (y[k].x != 0.0f || y[k].y != 0.0f) ++bad;
+                continue;
+            }
+            for (int t = 0; t < 2 * n; ++t) x[t] = in[st * pitch + (size_t)f * 2 * n + t];
+            orc_fft_r2c_f64(x.data(), 2 * n, ref.data());
+            double num = 0, den = 0;
+            for (int k = 0; k < n; ++k) {
+                const double dr = y[k].x - ref[2 * k], di = y[k].y - ref[2 * k + 1];
+                num += dr * dr + di * di;
+                den += ref[2 * k] * ref[2 * k] + ref[2 * k + 1] * ref[2 * k + 1];
+            }
+            const double e = std::sqrt(num / den);
+            if (e > worst) worst = e;
+            if (!(e < 1e-5)) ++bad;
+        }
+    std::printf("N=%d staged (%d x %d): worst relative RMS %.3e, %d bad\n", n, pl.m, pl.p, worst, bad);
+    return bad;
+}
+
 int main() {
     int bad = check_half();
     bad += check<9>() + check<10>() + check<11>() + check<12>() + check<13>() + check<14>();
     bad += check_mixed<3, 11>() + check_mixed<5, 11>() + check_mixed<3, 12>() + check_mixed<7, 11>();
     bad += check_unpack4();
+    bad += check_staged(18432) + check_staged(32768) + check_staged(45056);
     std::printf(bad ? "FAILED\n" : "OK\n");
     return bad ? 1 : 0;
 }

@@ -228,6 +228,7 @@ def main():
     ok = timed(golden, L, 4, 2048, 90, 32, 2)
     ok = timed(golden, L, 5, 4096, 37, 16, 3) and ok
     ok = timed(golden, L, 3, 6144, 20, 16, 1) and ok  # n_freq = 3 x 2048
+    ok = timed(golden, L, 2, 18432, 6, 4, 1) and ok   # 9 x 2048, staged through HBM
     ok = timed(packed, L, 3, 2048, 40, 16, 2) and ok
     ok = timed(skip, L) and ok
     ok = timed(forms, L, corr.FORM_FP16, 5, 512, 70, 32, 20) and ok

@@ -203,7 +203,8 @@ def _pipeline(corr, oracle, hip, form, N, nf, T, chunk, cal_frames, rho, seed, m
     return x, valid, gains, Yq, V, raw
 
 
-@pytest.mark.parametrize("form,nf", [(1, 2048), (2, 2048), (1, 6144), (2, 10240), (1, 14336), (2, 12288)])
+@pytest.mark.parametrize("form,nf", [(1, 2048), (2, 2048), (1, 6144), (2, 10240), (1, 14336), (2, 12288),
+                                     (1, 20480), (2, 32768)])
 def test_form_pipeline(corr, oracle, hip, form, nf):
     N, T, chunk = 6, 100, 32
     print(f"form {form} pipeline N={N} nf={nf} T={T} chunk={chunk}", flush=True)
@@ -281,7 +282,8 @@ def test_c3_shape_fp16_error(corr, oracle, hip):
     assert e32 < 1e-5
 
 
-@pytest.mark.parametrize("form,nf", [(1, 2048), (1, 16384), (2, 2048), (2, 16384), (1, 12288), (2, 14336)])
+@pytest.mark.parametrize("form,nf", [(1, 2048), (1, 16384), (2, 2048), (2, 16384), (1, 12288), (2, 14336),
+                                     (1, 18432), (2, 24576)])
 def test_fengine_form_rounding(corr, oracle, hip, form, nf):
     """The form's F-engine output is the FP32 form's spectrum y requantized on
     the host: FP16 = y*g rounded to nearest even (numpy float16), int8 =

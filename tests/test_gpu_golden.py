@@ -36,6 +36,8 @@ def golden_input(rows, N, n_cum, L, seed=20260914):
     (15, 4096, 200, 64, 2),   # golden_15st
     (4, 6144, 150, 64, 2),    # n_freq with an odd factor: 3 x 2048
     (3, 14336, 70, 32, 1),    # 7 x 2048
+    (2, 18432, 40, 16, 1),    # 9 x 2048, staged through HBM
+    (2, 32768, 20, 8, 1),     # 2 x 16384, staged
 ])
 def test_golden(corr, oracle, N, nf, n_cum, fps, rows):
     print(f"golden N={N} nf={nf} n_cum={n_cum} fps={fps} rows={rows}", flush=True)

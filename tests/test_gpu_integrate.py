@@ -26,7 +26,7 @@ def hip(corr):
 CASES = [(0, 6, 2048, 100, 32), (1, 6, 2048, 100, 32), (2, 6, 2048, 100, 32),
          (0, 3, 512, 5, 8), (1, 3, 512, 5, 8), (2, 40, 512, 77, 16),
          (1, 24, 16384, 80, 32), (2, 24, 16384, 80, 32), (0, 24, 16384, 48, 16),
-         (1, 8, 10240, 70, 32), (0, 5, 12288, 40, 16)]
+         (1, 8, 10240, 70, 32), (0, 5, 12288, 40, 16), (1, 4, 20480, 40, 16), (0, 3, 32768, 70, 32)]
 
 
 @pytest.mark.parametrize("form,N,nf,T,chunk", CASES)
