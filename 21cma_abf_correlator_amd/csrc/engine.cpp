@@ -58,10 +58,11 @@ int fft_twiddle_elems(int n_freq) {
     return p == n_freq ? n_freq : n_freq + p;
 }
 
-// Frames per batch of the staged transform: its two buffers then take
-// 2 * 64 * 8 bytes per input sample pair of a frame (at 24 inputs and n_freq
-// 65536, 1.6 GB).
+// Frames per batch of the staged transform: at most 64, and fewer where its two
+// buffers (8 bytes per retained channel, input and frame, each) would pass
+// 1 GiB each (at 24 inputs: 64 frames up to n_freq 65536, 5 at 2^20).
 constexpr int kStageFrames = 64;
+constexpr size_t kStageBytes = size_t(1) << 30;
 
 corr_status Engine::create(const EngineShape &shape, Engine **out) {
     if (out == nullptr) return fail(CORR_ERR_ARG, "engine: out is NULL");
@@ -71,7 +72,7 @@ corr_status Engine::create(const EngineShape &shape, Engine **out) {
     if (!fft_size_supported(shape.n_freq))
         return failf(CORR_ERR_NOT_IMPL,
                      "n_freq=%d: this build transforms n_freq = 2^k in [512, 16384], 6144, 10240, "
-                     "12288, 14336, and the multiples of 2048 up to 65536 (FFT length 2*n_freq)",
+                     "12288, 14336, and the multiples of 2048 up to 1048576 (FFT length 2*n_freq)",
                      shape.n_freq);
     if (shape.form != Form::kFP32 && shape.form != Form::kFP16 && shape.form != Form::kINT8)
         return failf(CORR_ERR_ARG, "engine: unknown form %d", (int)shape.form);
@@ -112,7 +113,10 @@ corr_status Engine::init() {
     if ((st = hip_check(hipMalloc(&d_spec_, spec_bytes), "hipMalloc(spectra)")) != CORR_OK)
         return st;
     if (fft::staged_plan(nf).m > 0) {
+        const size_t per_frame = sizeof(float2) * (size_t)N * (size_t)nf;
+        const size_t fit = kStageBytes / per_frame < 1 ? 1 : kStageBytes / per_frame;
         stage_frames_ = shape_.chunk_frames < kStageFrames ? shape_.chunk_frames : kStageFrames;
+        if ((size_t)stage_frames_ > fit) stage_frames_ = (int)fit;
         const size_t stage = sizeof(float2) * (size_t)N * (size_t)stage_frames_ * (size_t)nf;
         if ((st = hip_check(hipMalloc(reinterpret_cast<void **>(&d_stage_y_), stage), "hipMalloc(staged transform)")) !=
                 CORR_OK ||

@@ -49,7 +49,7 @@ constexpr int kMaxStationMfma = 48;
 /// FFT sizes this build transforms: n_freq = 2^k with 512 <= n_freq <= 16384;
 /// the multiples of 2048 below 16384 with an odd factor, 6144 (3 x 2048),
 /// 10240 (5 x 2048), 12288 (3 x 4096) and 14336 (7 x 2048), in one workgroup's
-/// LDS; and every multiple of 2048 from 18432 to 65536, staged through HBM.
+/// LDS; and every multiple of 2048 from 18432 to 1048576, staged through HBM.
 bool fft_size_supported(int n_freq);
 
 /// Twiddle table entries of a supported n_freq: exp(-2 pi i m/n_freq) for

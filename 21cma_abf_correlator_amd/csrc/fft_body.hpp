@@ -968,10 +968,11 @@ CORR_HD CORR_FI void fengine_mixed_form_block(int j, int f, int st, float2 *lds,
 }
 
 // ---------------------------------------------------------------------------
-// n_freq above 16384 (every multiple of 2048 up to 65536, which the reference
-// accepts, corr-cuda/src/corr_cuda.cpp:500-509): one frame's transform no
-// longer fits a workgroup's LDS, so it is staged through HBM in three kernels,
-// N = m P with P the largest power of two dividing N, at most 16384, and m <= 32:
+// n_freq above 16384 (every multiple of 2048 up to 2^20; the reference accepts
+// every multiple of 2048, corr-cuda/src/corr_cuda.cpp:500-509): one frame's
+// transform no longer fits a workgroup's LDS, so it is staged through HBM in
+// three kernels, N = m P with P the largest power of two dividing N, at most
+// 16384, and m = N / P (2 to 511; stage 1 costs m complex products per point):
 //   1. radix-m stage from the int8 input, one lane per (n1, k2):
 //        y_k2[n1] = W_N^(n1 k2) sum_n2 z[n1 + P n2] W_m^(n2 k2)
 //      with W_m^r = W_N^(r P), the phase n2 k2 reduced mod m exactly;
@@ -986,10 +987,13 @@ struct StagedPlan {
     int n = 0, m = 0, p = 0, log2p = 0;
 };
 
+// Largest n_freq of the staged transform: a frame of 2 MiB samples per input.
+constexpr int kMaxStagedFreq = 1 << 20;
+
 // The plan of an n_freq above 16384, or m = 0 where there is none.
 CORR_HD inline StagedPlan staged_plan(int n_freq) {
     StagedPlan s;
-    if (n_freq <= 16384 || n_freq > 65536 || n_freq % 2048 != 0) return s;
+    if (n_freq <= 16384 || n_freq > kMaxStagedFreq || n_freq % 2048 != 0) return s;
     int p = n_freq & -n_freq;
     if (p > 16384) p = 16384;
     s.n = n_freq;

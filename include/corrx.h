@@ -34,7 +34,7 @@ extern "C" {
 
 typedef struct corrx_shape {
     int32_t n_station;    /* inputs, >= 1                                    */
-    int32_t n_freq;       /* retained channels: 2^k in [512, 16384], or a multiple of 2048 up to 65536 */
+    int32_t n_freq;       /* retained channels: 2^k in [512, 16384], or a multiple of 2048 up to 1048576 */
     int32_t chunk_frames; /* most frames one fengine/xengine call handles    */
     int32_t form;         /* CORRX_FORM_*                                    */
 } corrx_shape;

@@ -435,7 +435,7 @@ int main() {
     bad += check<9>() + check<10>() + check<11>() + check<12>() + check<13>() + check<14>();
     bad += check_mixed<3, 11>() + check_mixed<5, 11>() + check_mixed<3, 12>() + check_mixed<7, 11>();
     bad += check_unpack4();
-    bad += check_staged(18432) + check_staged(32768) + check_staged(45056);
+    bad += check_staged(18432) + check_staged(32768) + check_staged(45056) + check_staged(67584);
     std::printf(bad ? "FAILED\n" : "OK\n");
     return bad ? 1 : 0;
 }

@@ -196,7 +196,7 @@ def test_pack4_layout(corr):
     ((49, 1024, 16, 2), "ERR_NOT_IMPL"),
     ((4, 1000, 16, 1), "ERR_NOT_IMPL"),   # FFT length
     ((4, 3072, 16, 0), "ERR_NOT_IMPL"),   # odd factor, not a multiple of 2048
-    ((4, 67584, 16, 2), "ERR_NOT_IMPL"),  # 33 * 2048, above 65536
+    ((4, 1050624, 16, 2), "ERR_NOT_IMPL"),  # 513 * 2048, above 2^20
     ((4, 20000, 16, 0), "ERR_NOT_IMPL"),  # above 16384 and not a multiple of 2048
     ((0, 1024, 16, 2), "ERR_ARG"),
     ((4, 1024, 0, 1), "ERR_ARG"),

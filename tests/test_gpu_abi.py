@@ -38,7 +38,7 @@ def test_device_dependent_rejections(corr):
         (dict(devices=[0, 64]), corr.ERR_ARG),
         (dict(devices=[0, 0]), corr.ERR_ARG),                  # a card driven twice
         (dict(n_station=45), corr.ERR_ARG),                    # 1035 > 1024 baselines
-        (dict(n_freq=67584), corr.ERR_NOT_IMPL),               # 33*2048: above 65536
+        (dict(n_freq=1050624), corr.ERR_NOT_IMPL),             # 513*2048: above 2^20
     ]:
         cfg = small_cfg(corr, **kw)
         h = ctypes.c_void_p(0x1)

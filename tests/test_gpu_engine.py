@@ -35,9 +35,10 @@ def _spectra(hip, eng, N, T, nf):
 
 
 # powers of two, the multiples of 2048 with an odd factor (3, 5, 3, 7), and
-# the staged lengths above 16384 (9 x 2048, 2 x 16384, 11 x 4096)
+# the staged lengths above 16384 (9 x 2048, 2 x 16384, 11 x 4096, 33 x 2048,
+# 8 x 16384, 255 x 4096)
 @pytest.mark.parametrize("nf", [512, 1024, 2048, 4096, 8192, 16384, 6144, 10240, 12288, 14336,
-                                18432, 32768, 45056])
+                                18432, 32768, 45056, 67584, 131072, 1044480])
 def test_fengine_vs_fp64(corr, oracle, hip, nf):
     N, T = 3, 4
     L = 2 * nf

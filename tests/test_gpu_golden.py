@@ -38,6 +38,7 @@ def golden_input(rows, N, n_cum, L, seed=20260914):
     (3, 14336, 70, 32, 1),    # 7 x 2048
     (2, 18432, 40, 16, 1),    # 9 x 2048, staged through HBM
     (2, 32768, 20, 8, 1),     # 2 x 16384, staged
+    (2, 67584, 12, 4, 1),     # 33 x 2048, staged, radix-33 first stage
 ])
 def test_golden(corr, oracle, N, nf, n_cum, fps, rows):
     print(f"golden N={N} nf={nf} n_cum={n_cum} fps={fps} rows={rows}", flush=True)
