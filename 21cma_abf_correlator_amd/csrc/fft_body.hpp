@@ -193,12 +193,35 @@ CORR_HD CORR_FI void group_twiddles(const float2 *tw, int k, float2 (&w)[R - 1])
     group_twiddle_products<R>(w);
 }
 
+// The twiddles of the second pass (R = 16, NS = 16) depend on the lane's group
+// k = j mod 16 alone: W_256^(k r) for r = 1..15, 240 values. The power-of-two
+// F-engine copies them from the FP64-built table into LDS, behind its exchange
+// buffer, before its first barrier; the pass then reads its 15 from one base
+// address, in place of 4 scattered table reads and 11 complex products per lane.
+constexpr int kTw16Elems = 16 * 15;
+
+template <int N>
+CORR_HD CORR_FI void fill_tw16(int j, float2 *tab, const float2 *tw) {
+    for (int i = j; i < kTw16Elems; i += N / 16) {
+        const int k = i / 15, r = i - 15 * k + 1;
+        tab[i] = tw[(unsigned)(k * r * (N / 256))];
+    }
+}
+
+CORR_HD CORR_FI void read_tw16(const float2 *tab, int j, float2 (&w)[15]) {
+    const float2 *t = tab + 15 * (j & 15);
+#pragma unroll
+    for (int r = 0; r < 15; ++r) w[r] = t[r];
+}
+
 // One Stockham pass of radix R over N points, NS = points already combined,
 // in place in `lds` (barrier between all reads and any write). Thread j does
 // 16/R butterflies jj = j + q*N/16. The pass's twiddles are requested before
 // its LDS reads and barrier, so their cache latency overlaps the exchange.
-template <int N, int R, int NS, class Sync>
-CORR_HD CORR_FI void stockham_pass(float2 (&v)[16], float2 *lds, const float2 *tw, int j, Sync &sync) {
+// With TAB, the second pass (NS = 16) reads its twiddles from `tab` (fill_tw16).
+template <int N, int R, int NS, bool TAB, class Sync>
+CORR_HD CORR_FI void stockham_pass(float2 (&v)[16], float2 *lds, const float2 *tw, const float2 *tab, int j,
+                                   Sync &sync) {
     constexpr int NT = N / 16;
     constexpr int NB = 16 / R;
     constexpr int STRIDE = N / R;
@@ -218,6 +241,8 @@ CORR_HD CORR_FI void stockham_pass(float2 (&v)[16], float2 *lds, const float2 *t
                 const int m = (2 * r * q) & 31;  // exp(-2 pi i m/32)
                 w[q][r - 1] = cmul(w[0][r - 1], make_float2(cos32(m), -cos32((m + 24) & 31)));
             }
+    } else if constexpr (TAB && NS == 16 && R == 16) {
+        read_tw16(tab, j, w[0]);
     } else {
 #pragma unroll
         for (int q = 0; q < NB; ++q) group_twiddles<R, TWSTEP>(tw, (j + q * NT) & (NS - 1), w[q]);
@@ -244,13 +269,14 @@ CORR_HD CORR_FI void stockham_pass(float2 (&v)[16], float2 *lds, const float2 *t
     sync();
 }
 
-template <int LOG2N, int DONE, class Sync>
-CORR_HD CORR_FI void stockham_rest(float2 (&v)[16], float2 *lds, const float2 *tw, int j, Sync &sync) {
+template <int LOG2N, int DONE, bool TAB, class Sync>
+CORR_HD CORR_FI void stockham_rest(float2 (&v)[16], float2 *lds, const float2 *tw, const float2 *tab, int j,
+                                   Sync &sync) {
     if constexpr (DONE < LOG2N) {
         constexpr int REM = LOG2N - DONE;
         constexpr int LOGR = REM >= 4 ? 4 : REM;
-        stockham_pass<(1 << LOG2N), (1 << LOGR), (1 << DONE)>(v, lds, tw, j, sync);
-        stockham_rest<LOG2N, DONE + LOGR>(v, lds, tw, j, sync);
+        stockham_pass<(1 << LOG2N), (1 << LOGR), (1 << DONE), TAB>(v, lds, tw, tab, j, sync);
+        stockham_rest<LOG2N, DONE + LOGR, TAB>(v, lds, tw, tab, j, sync);
     }
 }
 
@@ -332,26 +358,31 @@ CORR_HD CORR_FI bool frame_present(const int32_t *valid, size_t valid_stride, in
 }
 
 // The Z spectrum of thread j's converted input values into `lds` (natural
-// order, padded); on return `lds` holds Z after a final barrier.
-template <int LOG2N, class Sync>
-CORR_HD CORR_FI void fft_from_values(int j, float2 (&v)[16], float2 *lds, Sync &sync, const float2 *tw) {
+// order, padded); on return `lds` holds Z after a final barrier. With TAB the
+// second pass's twiddles are read from `tab`, filled before the first barrier.
+template <int LOG2N, bool TAB = false, class Sync>
+CORR_HD CORR_FI void fft_from_values(int j, float2 (&v)[16], float2 *lds, Sync &sync, const float2 *tw,
+                                     const float2 *tab = nullptr) {
     static_assert(LOG2N >= 9 && LOG2N <= 14, "the plain-layout R2C reads need a last pass with NS >= 256");
     // first pass: radix 16, NS = 1, inputs z[j + r*N/16] already in registers
     dft<16>(v);
 #pragma unroll
     for (int r = 0; r < 16; ++r) lds[17 * j + r] = v[r];  // padded layout: pad16(16 j + r)
     sync();
-    stockham_rest<LOG2N, 4>(v, lds, tw, j, sync);
+    stockham_rest<LOG2N, 4, TAB>(v, lds, tw, tab, j, sync);
 }
 
 // The Z spectrum of frame f of station st into `lds` (natural order, padded).
-// Thread j of N/16; on return `lds` holds Z after a final barrier.
+// Thread j of N/16; on return `lds` holds Z after a final barrier. `tab`
+// (kTw16Elems float2 past the exchange buffer) receives the second pass's
+// twiddles.
 template <int LOG2N, class Sync>
 CORR_HD CORR_FI void complex_fft_block(int j, int f, int st, float2 *lds, Sync &sync, const int8_t *in,
-                                      size_t station_pitch, const float2 *tw) {
+                                      size_t station_pitch, const float2 *tw, float2 *tab) {
     float2 v[16];
     load_frame<LOG2N>(j, in, station_pitch, f, st, v);
-    fft_from_values<LOG2N>(j, v, lds, sync, tw);
+    fill_tw16<(1 << LOG2N)>(j, tab, tw);
+    fft_from_values<LOG2N, true>(j, v, lds, sync, tw, tab);
 }
 
 // R2C bin k from Z in lds (plain layout: the last pass of every supported
@@ -630,18 +661,18 @@ CORR_HD CORR_FI void pass_twiddle_products(float2 (&w)[16 / R][R - 1]) {
 }
 
 // The N = 16384 transform (passes 16, 16, 16, 4) and the R2C epilogue, all
-// exchanges through the half buffer. emit(k, X) stores one bin.
+// exchanges through the half buffer `lds`. `tab` holds the second pass's
+// twiddles (fill_tw16, before the first barrier). emit(k, X) stores one bin.
 template <class Sync, class Emit>
-CORR_HD CORR_FI void fengine_half_16384(int j, float2 (&v)[16], float2 *lds, Sync &sync, const float2 *tw,
-                                        const float2 *wl, Emit &&emit) {
+CORR_HD CORR_FI void fengine_half_16384(int j, float2 (&v)[16], float2 *lds, const float2 *tab, Sync &sync,
+                                        const float2 *tw, const float2 *wl, Emit &&emit) {
     constexpr int N = 16384;
     constexpr int NT = N / 16;
     dft<16>(v);  // first pass: radix 16, NS = 1
     {
         float2 w[1][15];
-        pass_twiddle_reads<N, 16, 16>(tw, j, w);
         half_exchange<N, 16, 1, 16>(v, lds, j, sync, true);
-        pass_twiddle_products<N, 16, 16>(w);
+        read_tw16(tab, j, w[0]);
         pass_compute<N, 16, 16>(v, w);
     }
     {
@@ -699,9 +730,17 @@ CORR_HD CORR_FI void fengine_half_16384(int j, float2 (&v)[16], float2 *lds, Syn
 }
 
 // LDS elements (float2) of the F-engine workgroup at N = 2^LOG2N.
+// The power-of-two F-engine's LDS: the second pass's twiddle table
+// (kTw16Elems float2, first, so that its reads take small immediate offsets
+// from one address), then the exchange buffer.
+template <int LOG2N>
+CORR_HD constexpr int fengine_exchange_elems() {
+    return LOG2N == 14 ? HalfSizes<(1 << LOG2N)>::kLds : Sizes<(1 << LOG2N)>::kLds;
+}
+
 template <int LOG2N>
 CORR_HD constexpr int fengine_lds_elems() {
-    return LOG2N == 14 ? HalfSizes<(1 << LOG2N)>::kLds : Sizes<(1 << LOG2N)>::kLds;
+    return kTw16Elems + fengine_exchange_elems<LOG2N>();
 }
 
 // Lane j's bins of a missing frame, whose spectrum is zero, in the order the
@@ -734,13 +773,15 @@ CORR_HD CORR_FI void fengine_frame(int j, int f, int st, float2 *lds, Sync &sync
         emit_zero_frame<N>(j, emit);
         return;
     }
+    float2 *tab = lds, *xbuf = lds + kTw16Elems;
     if constexpr (LOG2N == 14) {
         float2 v[16];
         load_frame<LOG2N>(j, in, station_pitch, f, st, v);
-        fengine_half_16384(j, v, lds, sync, tw, wl, emit);
+        fill_tw16<N>(j, tab, tw);
+        fengine_half_16384(j, v, xbuf, tab, sync, tw, wl, emit);
     } else {
-        complex_fft_block<LOG2N>(j, f, st, lds, sync, in, station_pitch, tw);
-        r2c_epilogue<N>(j, lds, wl, emit);
+        complex_fft_block<LOG2N>(j, f, st, xbuf, sync, in, station_pitch, tw, tab);
+        r2c_epilogue<N>(j, xbuf, wl, emit);
     }
 }
 
