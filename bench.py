@@ -415,10 +415,15 @@ def main():
         for _ in range(args.warmup):
             step_serial()
             step()
-        serial_s = timed(step_serial, max(1, min(args.steps, 3)))
+        # the serial loop is timed before and after the timed steps and the two
+        # averaged: the clock drifts as the GPU stays loaded, and timing it on
+        # one side only would credit or charge the overlap with that drift
+        n_serial = max(1, min(args.steps, 3))
+        serial_before = timed(step_serial, n_serial)
         vis_serial = out.to_numpy(np.complex64, (B, nf))
         wall = timed(step, args.steps) * args.steps
         vis = out.to_numpy(np.complex64, (B, nf))
+        serial_s = 0.5 * (serial_before + timed(step_serial, n_serial))
         assert np.array_equal(vis.view(np.uint32), vis_serial.view(np.uint32)), \
             "overlapped and serial integrations differ"
         # Per-kernel durations from HIP events on the launch stream, each kernel
