@@ -126,6 +126,14 @@ CORR_HD inline void neumaier(float &s, float &e, float x) {
     s = t;
 }
 
+// The 16-byte slot of frame group o of channel ch in its staged rows is
+// o ^ slot_swizzle(ch). A wave's staging stores write the same row of eight
+// channels 4q + m (q < 8) at once, one channel pitch apart, and 4 pitches are
+// 64 bytes modulo the 256 bytes of the LDS banks, so without the swizzle
+// channels q and q + 4 fall in the same banks; the MFMA reads only need the
+// same slot order for both operands of a channel, which the swizzle keeps.
+CORR_HD inline int slot_swizzle(int ch) { return (ch & 16) ? 2 : 0; }
+
 // Staging of one item: 16 B (FP16) per frame for 8 frames, or 8 B (int8) per
 // frame for 16 frames, of one (input, channel quad), held in 32 registers
 // between its load and its store; the store transposes them into 8 rows of
@@ -188,7 +196,8 @@ struct Staging<1, CH> {  // FP16: 8 frames x 16 B (4 channels)
             im.y = (w[8] >> 16) | (w[12] & 0xFFFF0000u);
             im.z = (w[16] >> 16) | (w[20] & 0xFFFF0000u);
             im.w = (w[24] >> 16) | (w[28] & 0xFFFF0000u);
-            uint8_t *row = lds + (size_t)(4 * q + m) * cpitch + (size_t)(2 * s) * kRowBytes + 16 * o;
+            uint8_t *row = lds + (size_t)(4 * q + m) * cpitch + (size_t)(2 * s) * kRowBytes +
+                           16 * (o ^ slot_swizzle(4 * q + m));
             *reinterpret_cast<uint4 *>(row) = re;
             *reinterpret_cast<uint4 *>(row + kRowBytes) = im;
         }
@@ -236,7 +245,8 @@ struct Staging<2, CH> {  // int8: 16 frames x 8 B (4 channels)
                         acc |= ((st.w[2 * (4 * k + b) + (m >> 1)] >> sh) & 0xFFu) << (8 * b);
                     rw[k] = acc;
                 }
-                uint8_t *row = lds + (size_t)(4 * q + m) * cpitch + (size_t)(2 * s + p) * kRowBytes + 16 * o;
+                uint8_t *row = lds + (size_t)(4 * q + m) * cpitch + (size_t)(2 * s + p) * kRowBytes +
+                               16 * (o ^ slot_swizzle(4 * q + m));
                 *reinterpret_cast<uint4 *>(row) = r;
             }
         }
@@ -319,8 +329,9 @@ CORR_HD inline void xengine_mfma_block(int tid, int bx, int n_tiles, uint8_t *ld
         sync();
 #pragma unroll
         for (int ch = 0; ch < kChanPerWave; ++ch) {
-            const uint8_t *base = lds + (size_t)(w * kChanPerWave + ch) * cpitch +
-                                  (size_t)(lane & 15) * kRowBytes + 16 * (lane >> 4);
+            const int cl = w * kChanPerWave + ch;
+            const uint8_t *base = lds + (size_t)cl * cpitch + (size_t)(lane & 15) * kRowBytes +
+                                  16 * ((lane >> 4) ^ slot_swizzle(cl));
             Frag fr[NRT];
 #pragma unroll
             for (int a = 0; a < NRT; ++a) {
