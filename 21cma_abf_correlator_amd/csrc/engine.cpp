@@ -508,6 +508,12 @@ corr_status Engine::xengine_from(const void *spec, void *integ, int frames, hipS
 corr_status Engine::dump(const void *integ, uint64_t frames, float2 *out, hipStream_t s) {
     if (integ == nullptr || out == nullptr) return fail(CORR_ERR_ARG, "dump: NULL buffer");
     if (frames == 0) return fail(CORR_ERR_ARG, "dump: frames must be > 0");
+    // The int8 form's int32 integrator holds a frame's largest product sum
+    // 2 * 7 * 7 = 98 at most 2^31 / 98 times: longer integrations could have
+    // wrapped, so they are refused (the nominal count bounds the frames added).
+    if (shape_.form == Form::kINT8 && frames > kInt8MaxFrames)
+        return failf(CORR_ERR_ARG, "dump: the int8 form integrates at most %llu frames (int32 sums of products "
+                     "up to 98), got %llu", (unsigned long long)kInt8MaxFrames, (unsigned long long)frames);
     return hip_check(launch_dump((int)shape_.form, integ, n_baseline_, shape_.n_freq, d_pairs_, d_gain_,
                                  frames, out, s),
                      "dump_kernel");

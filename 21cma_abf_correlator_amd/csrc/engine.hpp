@@ -43,6 +43,10 @@ enum class Form : int {
 /// reference kernels.cu:108-110, 474-483).
 constexpr int kMaxBaselinesAbi = 1024;
 
+/// Longest integration of the int8 form: its int32 integrator adds at most
+/// 2 * 7 * 7 = 98 per frame (components clamped to +-7).
+constexpr uint64_t kInt8MaxFrames = 2147483647ull / 98ull;
+
 /// Largest input count of the FP16/int8 forms (2*n_station <= 96 Gram rows).
 constexpr int kMaxStationMfma = 48;
 

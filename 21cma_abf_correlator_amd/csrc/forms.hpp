@@ -100,6 +100,33 @@ CORR_HD inline uint32_t pack_fp16(float2 y, float g) {
 #endif
 }
 
+// Two IEEE halves (one 32-bit word) with each infinity replaced by the largest
+// finite half of its sign, +-65504; finite values are unchanged. The FP16 form's
+// F-engine converts y*g with round to nearest even, so a bin beyond the half
+// range (|y*g| >= 65520, i.e. 48 dB above the calibrated RMS of 256) is stored
+// as +-inf; the X-engine reads every spectrum word through this function, which
+// gives exactly the products of y*g clamped to +-65504 before the conversion
+// (the values in [65504, 65520) round to 65504 either way). NaN cannot occur:
+// y and g are finite.
+CORR_HD inline uint32_t saturate_fp16x2(uint32_t w) {
+    // On the bit patterns: the magnitude of each half (sign cleared) is capped
+    // at 0x7BFF = 65504 and the sign put back; inf (0x7C00) is the only
+    // magnitude above it that can occur. Integer ops: exact, no canonicalisation.
+#if defined(__HIP_DEVICE_COMPILE__)
+    typedef unsigned short u16x2_t __attribute__((ext_vector_type(2)));
+    const u16x2_t cap = {(unsigned short)0x7BFFu, (unsigned short)0x7BFFu};
+    const u16x2_t mag = __builtin_bit_cast(u16x2_t, w & 0x7FFF7FFFu);
+    return (w & 0x80008000u) | __builtin_bit_cast(uint32_t, __builtin_elementwise_min(mag, cap));
+#else
+    uint32_t out = w & 0x80008000u;
+    for (int h = 0; h < 2; ++h) {
+        const uint32_t m = (w >> (16 * h)) & 0x7FFFu;
+        out |= (m < 0x7BFFu ? m : 0x7BFFu) << (16 * h);
+    }
+    return out;
+#endif
+}
+
 CORR_HD inline int quant_int8(float v) {
     const float r = rintf(v);
     return r > (float)kInt8Max ? kInt8Max : (r < -(float)kInt8Max ? -kInt8Max : (int)r);

@@ -11,7 +11,9 @@
 // y = Y_k; corr-cuda/src/kernels.cu:243-246):
 //   Re V_jk = M[2k][2j] + M[2k+1][2j+1],   Im V_jk = M[2k][2j+1] - M[2k+1][2j]
 // Products of two FP16 values are exact in FP32 and those of two int8 values
-// exact in int32; the chunk's sum is added into the integrator once per call,
+// exact in int32; FP16 words are saturated to +-65504 as they are staged
+// (forms::saturate_fp16x2), so a bin that overflowed the half range counts as
+// the largest finite half, never as inf; the chunk's sum is added into the integrator once per call,
 // with Neumaier-compensated FP32 (FP16 form) or exactly in int32 (int8 form).
 //
 // Workgroup: CH = 16 or 32 channels, CH/2 waves, 2 channels per wave. One step
@@ -70,6 +72,13 @@ struct Acc {
 /// (spectrum word = 2 int8 in 16 bits).
 template <int FORM>
 struct Traits;
+
+template <>
+struct Traits<0> {  // FP32: complex64 spectra, 4 x v_mfma_f32_16x16x4_f32 per 16 frames
+    using acc_t = float;
+    static constexpr int kFramesPerStep = 16;
+    static constexpr int kElemBytes = 8;
+};
 
 template <>
 struct Traits<1> {  // FP16
@@ -157,6 +166,54 @@ template <int FORM, int CH>
 struct Staging;
 
 template <int CH>
+struct Staging<0, CH> {  // FP32: 4 frames x 32 B (4 complex64 channels)
+    static CORR_HD inline void load(int item, const void *spectra, int nf, int fcap, int tile, int frames,
+                                    int f0, int c0, Stage &st) {
+        int q, o, s;
+        item_coords<CH>(item, q, o, s);
+        // frames f0 + 4o .. +3 of channels c0 + 4q .. +3: rows `tile` elements apart
+        const uint32_t *y = static_cast<const uint32_t *>(spectra) +
+                            2 * forms::spec_index(s, f0 + 4 * o, c0 + 4 * q, nf, fcap, tile);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int f = f0 + 4 * o + i;
+            if (f < frames) {
+                const uint4 *src = reinterpret_cast<const uint4 *>(y + 2 * (size_t)i * (size_t)tile);
+                const uint4 a = src[0], b = src[1];
+                st.w[8 * i + 0] = a.x;  // re, im of channel 0
+                st.w[8 * i + 1] = a.y;
+                st.w[8 * i + 2] = a.z;  // channel 1
+                st.w[8 * i + 3] = a.w;
+                st.w[8 * i + 4] = b.x;  // channel 2
+                st.w[8 * i + 5] = b.y;
+                st.w[8 * i + 6] = b.z;  // channel 3
+                st.w[8 * i + 7] = b.w;
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) st.w[8 * i + e] = 0u;
+            }
+        }
+    }
+
+    // Row 2s of channel m holds the real parts of frames 4o..4o+3, row 2s+1 the
+    // imaginary parts: w[8i + 2m] and w[8i + 2m + 1] of frame i, no arithmetic.
+    static CORR_HD inline void store(int item, uint8_t *lds, size_t cpitch, const Stage &st) {
+        int q, o, s;
+        item_coords<CH>(item, q, o, s);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const uint4 re = make_uint4(st.w[2 * m], st.w[8 + 2 * m], st.w[16 + 2 * m], st.w[24 + 2 * m]);
+            const uint4 im =
+                make_uint4(st.w[2 * m + 1], st.w[8 + 2 * m + 1], st.w[16 + 2 * m + 1], st.w[24 + 2 * m + 1]);
+            uint8_t *row = lds + (size_t)(4 * q + m) * cpitch + (size_t)(2 * s) * kRowBytes +
+                           16 * (o ^ slot_swizzle(4 * q + m));
+            *reinterpret_cast<uint4 *>(row) = re;
+            *reinterpret_cast<uint4 *>(row + kRowBytes) = im;
+        }
+    }
+};
+
+template <int CH>
 struct Staging<1, CH> {  // FP16: 8 frames x 16 B (4 channels)
     static CORR_HD inline void load(int item, const void *spectra, int nf, int fcap, int tile, int frames,
                                     int f0, int c0, Stage &st) {
@@ -196,6 +253,16 @@ struct Staging<1, CH> {  // FP16: 8 frames x 16 B (4 channels)
             im.y = (w[8] >> 16) | (w[12] & 0xFFFF0000u);
             im.z = (w[16] >> 16) | (w[20] & 0xFFFF0000u);
             im.w = (w[24] >> 16) | (w[28] & 0xFFFF0000u);
+            // here, where the staged words are consumed, not at their load:
+            // the loads of the next step stay in flight during the MFMA phase
+            re.x = forms::saturate_fp16x2(re.x);
+            re.y = forms::saturate_fp16x2(re.y);
+            re.z = forms::saturate_fp16x2(re.z);
+            re.w = forms::saturate_fp16x2(re.w);
+            im.x = forms::saturate_fp16x2(im.x);
+            im.y = forms::saturate_fp16x2(im.y);
+            im.z = forms::saturate_fp16x2(im.z);
+            im.w = forms::saturate_fp16x2(im.w);
             uint8_t *row = lds + (size_t)(4 * q + m) * cpitch + (size_t)(2 * s) * kRowBytes +
                            16 * (o ^ slot_swizzle(4 * q + m));
             *reinterpret_cast<uint4 *>(row) = re;

@@ -44,6 +44,9 @@ INT8_MFMA_PEAK_TOPS = 5000.0
 FORMS = {"fp32": 0, "fp16": 1, "int8": 2}
 SPEC_BYTES = {"fp32": 8, "fp16": 4, "int8": 2}      # per complex spectrum value
 INTEG_BYTES = {"fp32": 8, "fp16": 16, "int8": 8}    # per integrator element
+REQUANT = {"fp32": "none: complex64 spectra",
+           "fp16": "IEEE half per component, power-of-two gain per (input, channel), saturated at +-65504",
+           "int8": "4-bit: each component rint(y*g) clamped to +-7 (15 levels), held as int8"}
 
 
 def parse():
@@ -129,6 +132,19 @@ def rel_rms(got, ref, rows=None):
     if rows is not None:
         got, ref = got[rows], ref[rows]
     return float(np.sqrt(np.sum(np.abs(got - ref) ** 2) / np.sum(np.abs(ref) ** 2)))
+
+
+def per_baseline_error(got, ref, N):
+    """Relative RMS error of every visibility product (baseline) over its
+    channels against FP64: the worst baseline, its (j, k), and the median."""
+    import numpy as np
+    num = np.sum(np.abs(got.astype(np.complex128) - ref) ** 2, axis=1)
+    den = np.sum(np.abs(ref) ** 2, axis=1)
+    e = np.sqrt(num / np.where(den > 0, den, 1.0))
+    b = int(np.argmax(e))
+    j, k = np.triu_indices(N)
+    return {"worst": float(e[b]), "worst_baseline": b, "worst_pair": [int(j[b]), int(k[b])],
+            "median": float(np.median(e)), "baselines": int(e.size)}
 
 
 class ProfilerRegion:
@@ -240,7 +256,9 @@ def batch_c5(corr, hip, args, fp64):
              "integration_ms_realtime": round(s["T"] * s["L"] / 800e6 * 1e3, 3)}
         if fp64:
             V = s["out"].to_numpy(np.complex64, (s["B"], s["nf"]))
-            d["rel_rms_vs_fp64"] = rel_rms(V, fp64_visibilities(s["pool"]))
+            ref = fp64_visibilities(s["pool"])
+            d["rel_rms_vs_fp64"] = rel_rms(V, ref)
+            d["rel_rms_vs_fp64_per_baseline"] = per_baseline_error(V, ref, s["N"])
         rep.append(d)
     batch.close()
     for s in subs:
@@ -501,6 +519,8 @@ def main():
             d["rel_rms_vs_fp64"] = rel_rms(r["vis"], ref)
             if len(cross):
                 d["rel_rms_vs_fp64_cross"] = rel_rms(r["vis"], ref, cross)
+            d["rel_rms_vs_fp64_per_baseline"] = per_baseline_error(r["vis"], ref, N)
+        d["requantization"] = REQUANT[name]
         if name != "fp32" and "fp32" in res:
             d["rel_rms_vs_fp32_form"] = rel_rms(r["vis"], res["fp32"]["vis"].astype(np.complex128))
             # per frame: the forms may launch different frame counts

@@ -117,7 +117,9 @@ corr_status corrx_integrator_bytes(void *engine, size_t *bytes);
 /* Visibilities of an integration of `frames` frames (the caller's nominal count,
    as corr_row_end): d_out[b*n_freq + c] complex64, the dump layout of
    corr_rows_fetch. FP32: d_acc * (1.0f/(float)frames) (d_out may equal d_acc);
-   FP16/INT8: integrator / (g_j[c] g_k[c] frames), evaluated in FP64. */
+   FP16/INT8: integrator / (g_j[c] g_k[c] frames), evaluated in FP64.
+   INT8: CORR_ERR_ARG for frames > 2^31/98 = 21913098, the most its int32
+   integrator holds without wrapping. */
 corr_status corrx_dump(void *engine, const void *d_acc, uint64_t frames, void *d_out, void *stream);
 
 /* FP16/INT8 forms: set the per-(input, channel) gains from the mean power of

@@ -112,6 +112,31 @@ def test_generator_pinned_and_plausible(oracle):
     assert abs(r - 0.3) < 0.02
 
 
+def test_generator_tone_closed_form(oracle):
+    """The CW tone of the C2 workload, pinned by its spectrum rather than by a
+    hash the oracle wrote: with no noise, A cos(2 pi c t / L) rounded to int8 is
+    one line at bin c of height A L / 2 (numpy's FFT, within the rounding's
+    share), identical on every input and in every frame, and nothing elsewhere
+    above the rounding noise; with noise, the tone adds (A L / 2)^2 to the mean
+    power of bin c, on top of the noise's sigma^2 L."""
+    L, c, A = 2048, 37, 100.0
+    x = oracle.gen_int8(5, 3, 4, L, 0.3, sigma=0.0, max_delay=4, tone_channel=c, tone_amp=A)
+    assert np.array_equal(x[0], x[2]) and np.array_equal(x[0, 0], x[0, 3])
+    Y = np.fft.rfft(x[0, 0].astype(np.float64))
+    assert np.argmax(np.abs(Y)) == c
+    assert abs(abs(Y[c]) / (A * L / 2) - 1.0) < 1e-3
+    others = np.abs(np.delete(Y, c))
+    assert others.max() < 1e-3 * abs(Y[c])
+    # the C2 tone in noise, at the C2 geometry's channel and amplitude
+    L, c, A = 16384, 3000, 8.0
+    x = oracle.gen_int8(2, 2, 16, L, 0.3, sigma=16.0, max_delay=32, tone_channel=c, tone_amp=A)
+    P = (np.abs(np.fft.rfft(x.astype(np.float64), axis=2)) ** 2).mean(axis=1)   # (inputs, bins)
+    assert np.all(np.argmax(P, axis=1) == c)
+    want = (A * L / 2) ** 2 + 16.0 ** 2 * L
+    assert np.all(np.abs(P[:, c] / want - 1.0) < 0.05)
+    assert abs(np.median(P[:, 100:2900]) / (16.0 ** 2 * L) - 1.0) < 0.1
+
+
 def test_delay_phase_slope(oracle):
     # tools/phase_check.py:4-15 closed form: x1[t] = x0[t - tau] gives
     # phase(V_01[c]) = 2 pi c tau / L. rho = 1 makes x_i = s[t - d_i] exactly.
