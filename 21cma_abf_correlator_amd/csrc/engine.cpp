@@ -169,17 +169,25 @@ corr_status Engine::init() {
         std::memcpy(&one_bits, &one, 4);
         if ((st = hip_check(launch_fill(d_gain_, g, one_bits, nullptr), "fill_kernel(gains)")) != CORR_OK)
             return st;
+    }
+    // Every form runs its X-engine on the matrix cores up to 48 inputs (96 Gram
+    // rows); the FP32 form beyond that keeps the vector X-engine.
+    x_mfma_ = N <= kMaxStationMfma;
+    if (!x_mfma_) {
+        x_tile_ = 0;  // reported as "no tile": the vector X-engine
+        spec_tile_ = nf;
+    }
+    if (x_mfma_)
         for (int ch : {16, 32})
             if (xmfma::tile_fits(N, ch) && nf % ch == 0 &&
                 (st = hip_check(xengine_mfma_prepare((int)shape_.form, N, ch),
                                 "hipFuncSetAttribute(xengine_mfma)")) != CORR_OK)
                 return st;
-    }
 
     if ((st = hip_check(launch_twiddles(d_tw_, d_wl_, nf, nullptr), "twiddle_kernel")) != CORR_OK)
         return st;
     if ((st = hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize(twiddles)")) != CORR_OK) return st;
-    return shape_.form == Form::kFP32 ? CORR_OK : tune_xtile();
+    return x_mfma_ ? tune_xtile() : CORR_OK;
 }
 
 namespace {
@@ -220,11 +228,13 @@ corr_status Engine::tune_xtile() {
     spec_tile_ = cand[0].second;
     // CORRX_XTILE=<channels>,<spectrum tile> (tile 0: n_freq, the plain order)
     // pins the choice wherever it fits the shape, with no timing at creation;
-    // where it does not fit, the tuner runs.
+    // where it does not fit, the tuner runs. "0,0" pins the FP32 form's vector
+    // X-engine.
     if (const char *env = std::getenv("CORRX_XTILE")) {
         int ch = 0, tl = -1;
         if (std::sscanf(env, "%d,%d", &ch, &tl) == 2) {
             if (tl == 0) tl = nf;
+            if (ch == 0 && shape_.form == Form::kFP32) return set_xtile(0, nf);
             for (const auto &c : cand)
                 if (c.first == ch && c.second == tl) {
                     x_tile_ = ch;
@@ -306,13 +316,23 @@ corr_status Engine::tune_xtile() {
 }
 
 corr_status Engine::set_xtile(int channels, int spec_tile) {
-    if (shape_.form == Form::kFP32) return fail(CORR_ERR_ARG, "set_xtile: the FP32 form has no MFMA tile");
+    if (shape_.form == Form::kFP32 && channels == 0 && spec_tile == shape_.n_freq) {
+        // the FP32 form's vector X-engine, on spectra in the plain order
+        x_mfma_ = false;
+        x_tile_ = 0;
+        spec_tile_ = shape_.n_freq;
+        return CORR_OK;
+    }
+    if (shape_.n_station > kMaxStationMfma)
+        return failf(CORR_ERR_ARG, "set_xtile: %d inputs take the vector X-engine, which has no tile",
+                     shape_.n_station);
     if (!xmfma::tile_fits(shape_.n_station, channels) || shape_.n_freq % channels != 0)
         return failf(CORR_ERR_ARG, "set_xtile: %d channels per workgroup do not fit %d inputs / n_freq %d",
                      channels, shape_.n_station, shape_.n_freq);
     if (spec_tile != channels && spec_tile != shape_.n_freq)
         return failf(CORR_ERR_ARG, "set_xtile: the spectrum tile must be %d or n_freq (%d), not %d", channels,
                      shape_.n_freq, spec_tile);
+    x_mfma_ = true;
     x_tile_ = channels;
     spec_tile_ = spec_tile;
     return CORR_OK;
@@ -483,7 +503,7 @@ corr_status Engine::transform(int form, const int8_t *d_in, size_t station_pitch
     }
     if (form == 0)
         return hip_check(launch_fengine_fp32(nf, d_in, station_pitch, valid, valid_stride, d_tw_, d_wl_,
-                                             static_cast<float2 *>(out), out_station_stride, frames, N, s),
+                                             static_cast<float2 *>(out), out_station_stride, tile, frames, N, s),
                          "fengine_fp32_kernel");
     return hip_check(launch_fengine_form(form, nf, d_in, station_pitch, valid, valid_stride, d_tw_, d_wl_, d_gain_,
                                          out, out_station_stride, tile, frames, N, s),
@@ -494,7 +514,7 @@ corr_status Engine::xengine_from(const void *spec, void *integ, int frames, hipS
     if (integ == nullptr) return fail(CORR_ERR_ARG, "xengine: integrator is NULL");
     if (frames < 0 || frames > shape_.chunk_frames)
         return failf(CORR_ERR_ARG, "xengine: frames=%d outside [0, %d]", frames, shape_.chunk_frames);
-    if (shape_.form == Form::kFP32)
+    if (!x_mfma_)
         return hip_check(launch_xengine_fp32(static_cast<const float2 *>(spec), spectra_station_stride(),
                                              shape_.n_freq, frames, shape_.n_station, d_bpairs_, n_bpairs_,
                                              x_npair_, x_waves_, x_groups_, x_frames_lds_,

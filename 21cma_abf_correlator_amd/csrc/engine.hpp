@@ -181,6 +181,7 @@ class Engine {
     int n_baseline_ = 0;
 
     int cal_frames_ = 1;        ///< frames per calibration batch (FP32 spectra in d_spec_)
+    bool x_mfma_ = true;        ///< X-engine on the matrix cores (up to 48 inputs)
     int x_tile_ = 16;           ///< channels per MFMA X-engine workgroup
     int spec_tile_ = 16;        ///< channel tile of the spectrum order (x_tile_ or n_freq)
     corr_status tune_xtile();
@@ -223,9 +224,10 @@ hipError_t launch_unpack4(const int8_t *in, size_t in_pitch, int8_t *out, size_t
 hipError_t launch_twiddles(float2 *tw, float2 *wl, int n_freq, hipStream_t s);
 hipError_t launch_validity(const uint8_t *mask, size_t mask_pitch, int bit0, int frames,
                            int n_station, int32_t *valid, size_t valid_stride, hipStream_t s);
+/// tile: channels of the spectrum tile, 16, 32 or n_freq (the plain order).
 hipError_t launch_fengine_fp32(int n_freq, const int8_t *in, size_t station_pitch,
                                const int32_t *valid, size_t valid_stride, const float2 *tw,
-                               const float2 *wl, float2 *out, size_t out_station_stride,
+                               const float2 *wl, float2 *out, size_t out_station_stride, int tile,
                                int frames, int n_station, hipStream_t s);
 hipError_t launch_xengine_fp32(const float2 *y, size_t y_station_stride, int n_freq,
                                int frames, int n_station, const int2 *bpairs, int n_bpairs,

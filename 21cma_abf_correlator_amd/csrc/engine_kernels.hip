@@ -59,6 +59,17 @@ struct MfmaOps {
             __builtin_bit_cast(intx4_t, c), 0, 0, 0);
         return __builtin_bit_cast(xmfma::Acc<int32_t>, d);
     }
+    // D = A B + C as four 16x16x4 FP32 -> FP32 products in turn, the i-th
+    // taking word i of each lane's fragments (one float of A and of B per lane)
+    __device__ xmfma::Acc<float> mma_f32x4(const xmfma::Frag &a, const xmfma::Frag &b,
+                                           const xmfma::Acc<float> &c) const {
+        floatx4_t d = __builtin_bit_cast(floatx4_t, c);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            d = __builtin_amdgcn_mfma_f32_16x16x4f32(__builtin_bit_cast(float, a.w[i]),
+                                                     __builtin_bit_cast(float, b.w[i]), d, 0, 0, 0);
+        return __builtin_bit_cast(xmfma::Acc<float>, d);
+    }
     // value of lane l^1
     __device__ float xor1(float x) const { return __shfl_xor(x, 1); }
     __device__ int32_t xor1(int32_t x) const { return __shfl_xor(x, 1); }
@@ -77,12 +88,12 @@ __attribute__((amdgpu_waves_per_eu(LOG2N == 14 ? 8 : 1)))
 fengine_fp32_kernel(const int8_t *__restrict__ in, size_t station_pitch,
                     const int32_t *__restrict__ valid, size_t valid_stride,
                     const float2 *__restrict__ tw, const float2 *__restrict__ wl,
-                    float2 *__restrict__ out, size_t out_station_stride) {
+                    float2 *__restrict__ out, size_t out_station_stride, int tile_shift) {
     __shared__ float2 lds[fft::fengine_lds_elems<LOG2N>()];
     DeviceSync sync;
     fft::fengine_fp32_block<LOG2N>((int)threadIdx.x, (int)blockIdx.x, (int)blockIdx.y, lds, sync, in,
                                    station_pitch, valid, valid_stride, tw, wl, out,
-                                   out_station_stride);
+                                   out_station_stride, tile_shift);
 }
 
 // F-engine of the FP16 (FORM 1) and int8 (FORM 2) forms: as above, then gain
@@ -110,11 +121,12 @@ __global__ void __launch_bounds__((M << LOG2P) / 16)
 fengine_mixed_fp32_kernel(const int8_t *__restrict__ in, size_t station_pitch,
                           const int32_t *__restrict__ valid, size_t valid_stride,
                           const float2 *__restrict__ tw, const float2 *__restrict__ wl,
-                          float2 *__restrict__ out, size_t out_station_stride) {
+                          float2 *__restrict__ out, size_t out_station_stride, int tile) {
     __shared__ float2 lds[fft::MixedSizes<M, LOG2P>::kLds];
     DeviceSync sync;
     fft::fengine_mixed_fp32_block<M, LOG2P>((int)threadIdx.x, (int)blockIdx.x, (int)blockIdx.y, lds, sync, in,
-                                            station_pitch, valid, valid_stride, tw, wl, out, out_station_stride);
+                                            station_pitch, valid, valid_stride, tw, wl, out, out_station_stride,
+                                            tile);
 }
 
 template <int M, int LOG2P, int FORM>
@@ -159,9 +171,9 @@ staged_sub_kernel(const int32_t *__restrict__ valid, size_t valid_stride, int fr
                            valid, valid_stride, lds, sync, ys, twp, zs);
 }
 
-// Stage 3: grid (ceil((N/2+1)/256), frames, stations). FORM 0: complex64
-// [station][frame][N]; FORM 1/2: the form's words in the tile-major order of
-// forms::spec_index (tile 16, 32 or N), bin k scaled by gain[st*N + k].
+// Stage 3: grid (ceil((N/2+1)/256), frames, stations). Every form's words in
+// the order of forms::spec_index (tile 16, 32 or N, the plain order): FORM 0
+// complex64, FORM 1/2 bin k scaled by gain[st*N + k] and requantized.
 template <int FORM>
 __global__ void __launch_bounds__(kStagedBlock)
 staged_r2c_kernel(const int32_t *__restrict__ valid, size_t valid_stride, int frame0, int frames, int n, int m,
@@ -176,10 +188,10 @@ staged_r2c_kernel(const int32_t *__restrict__ valid, size_t valid_stride, int fr
     const int fcap = (int)(out_station_stride / (size_t)n);
     auto at = [&](int k) { return forms::spec_index(st, frame, k, n, fcap, tile); };
     if constexpr (FORM == 0) {
-        float2 *o = static_cast<float2 *>(out) + (size_t)st * out_station_stride + (size_t)frame * (size_t)n;
+        float2 *o = static_cast<float2 *>(out);
         fft::staged_r2c((int)(blockIdx.x * kStagedBlock + threadIdx.x), f, st, frames, pl, frame0, valid,
-                        valid_stride, zs, wl, [&](int k, float2 x) { o[k] = x; },
-                        [&](int k) { o[k] = make_float2(0.0f, 0.0f); });
+                        valid_stride, zs, wl, [&](int k, float2 x) { o[at(k)] = x; },
+                        [&](int k) { o[at(k)] = make_float2(0.0f, 0.0f); });
     } else if constexpr (FORM == 1) {
         uint32_t *o = static_cast<uint32_t *>(out);
         const float *g = gain + (size_t)st * (size_t)n;
@@ -363,11 +375,12 @@ hipError_t launch_fengine_mixed_form_t(const int8_t *in, size_t station_pitch, c
 template <int M, int LOG2P>
 hipError_t launch_fengine_mixed_fp32_t(const int8_t *in, size_t station_pitch, const int32_t *valid,
                                        size_t valid_stride, const float2 *tw, const float2 *wl, float2 *out,
-                                       size_t out_station_stride, int frames, int n_station, hipStream_t s) {
+                                       size_t out_station_stride, int tile, int frames, int n_station,
+                                       hipStream_t s) {
     const dim3 grid((unsigned)frames, (unsigned)n_station);
     const dim3 block((unsigned)fft::MixedSizes<M, LOG2P>::kThreads);
     hipLaunchKernelGGL((fengine_mixed_fp32_kernel<M, LOG2P>), grid, block, 0, s, in, station_pitch, valid,
-                       valid_stride, tw, wl, out, out_station_stride);
+                       valid_stride, tw, wl, out, out_station_stride, tile);
     return hipGetLastError();
 }
 
@@ -460,12 +473,12 @@ hipError_t launch_xengine_mfma_f(int n_station, const void *spectra, size_t ys, 
 template <int LOG2N>
 hipError_t launch_fengine_fp32_t(const int8_t *in, size_t station_pitch, const int32_t *valid,
                                  size_t valid_stride, const float2 *tw, const float2 *wl,
-                                 float2 *out, size_t out_station_stride, int frames,
+                                 float2 *out, size_t out_station_stride, int tile_shift, int frames,
                                  int n_station, hipStream_t s) {
     const dim3 grid((unsigned)frames, (unsigned)n_station);
     const dim3 block((1u << LOG2N) / 16u);
     hipLaunchKernelGGL(fengine_fp32_kernel<LOG2N>, grid, block, 0, s, in, station_pitch, valid,
-                       valid_stride, tw, wl, out, out_station_stride);
+                       valid_stride, tw, wl, out, out_station_stride, tile_shift);
     return hipGetLastError();
 }
 
@@ -473,14 +486,17 @@ hipError_t launch_fengine_fp32_t(const int8_t *in, size_t station_pitch, const i
 
 hipError_t launch_fengine_fp32(int n_freq, const int8_t *in, size_t station_pitch,
                                const int32_t *valid, size_t valid_stride, const float2 *tw,
-                               const float2 *wl, float2 *out, size_t out_station_stride,
+                               const float2 *wl, float2 *out, size_t out_station_stride, int tile,
                                int frames, int n_station, hipStream_t s) {
     if (frames <= 0 || n_station <= 0) return hipSuccess;
+    // a tile of n_freq channels is the plain order [station][frame][channel]
+    if (tile != 16 && tile != 32 && tile != n_freq) return hipErrorInvalidValue;
     switch (n_freq) {
 #define CORR_FENGINE_CASE(L)                                                                      \
     case 1 << L:                                                                                  \
         return launch_fengine_fp32_t<L>(in, station_pitch, valid, valid_stride, tw, wl, out, \
-                                        out_station_stride, frames, n_station, s);
+                                        out_station_stride, tile == 16 ? 4 : tile == 32 ? 5 : L,  \
+                                        frames, n_station, s);
         CORR_FENGINE_CASE(9)
         CORR_FENGINE_CASE(10)
         CORR_FENGINE_CASE(11)
@@ -491,7 +507,7 @@ hipError_t launch_fengine_fp32(int n_freq, const int8_t *in, size_t station_pitc
 #define CORR_FMIXED_CASE(M, L)                                                                                \
     case M << L:                                                                                              \
         return launch_fengine_mixed_fp32_t<M, L>(in, station_pitch, valid, valid_stride, tw, wl, out, \
-                                                 out_station_stride, frames, n_station, s);
+                                                 out_station_stride, tile, frames, n_station, s);
         CORR_FMIXED_CASE(3, 11)
         CORR_FMIXED_CASE(5, 11)
         CORR_FMIXED_CASE(3, 12)
@@ -508,7 +524,7 @@ hipError_t launch_fengine_staged(int form, int n_freq, const int8_t *in, size_t 
                                  float2 *ys, float2 *zs, hipStream_t s) {
     if (frames <= 0 || n_station <= 0) return hipSuccess;
     const fft::StagedPlan pl = fft::staged_plan(n_freq);
-    if (pl.m == 0 || form < 0 || form > 2 || (form != 0 && tile != 16 && tile != 32 && tile != n_freq) ||
+    if (pl.m == 0 || form < 0 || form > 2 || (tile != 16 && tile != 32 && tile != n_freq) ||
         (size_t)frames * (size_t)n_station > 65535u)
         return hipErrorInvalidValue;
     hipLaunchKernelGGL(staged_first_kernel, dim3((unsigned)((pl.p + kStagedBlock - 1) / kStagedBlock), (unsigned)pl.m,
@@ -650,6 +666,7 @@ hipError_t xengine_mfma_prepare(int form, int n_station, int ch) {
     if (!xmfma::tile_fits(n_station, ch)) return hipErrorInvalidValue;
     const int nrt = xmfma::row_tiles(n_station);
     const void *fn = nullptr;
+    if (form == 0) fn = ch == 16 ? xengine_mfma_fn<0, 16>(nrt) : xengine_mfma_fn<0, 32>(nrt);
     if (form == 1) fn = ch == 16 ? xengine_mfma_fn<1, 16>(nrt) : xengine_mfma_fn<1, 32>(nrt);
     if (form == 2) fn = ch == 16 ? xengine_mfma_fn<2, 16>(nrt) : xengine_mfma_fn<2, 32>(nrt);
     if (fn == nullptr) return hipErrorInvalidValue;
@@ -662,6 +679,9 @@ hipError_t launch_xengine_mfma(int form, int n_station, int ch, const void *spec
     if (frames <= 0) return hipSuccess;
     if (!xmfma::tile_fits(n_station, ch) || nf % ch != 0) return hipErrorInvalidValue;
     if (spec_tile != ch && spec_tile != nf) return hipErrorInvalidValue;
+    if (form == 0)
+        return ch == 16 ? launch_xengine_mfma_f<0, 16>(n_station, spectra, ys, nf, spec_tile, frames, integ, s)
+                        : launch_xengine_mfma_f<0, 32>(n_station, spectra, ys, nf, spec_tile, frames, integ, s);
     if (form == 1)
         return ch == 16 ? launch_xengine_mfma_f<1, 16>(n_station, spectra, ys, nf, spec_tile, frames, integ, s)
                         : launch_xengine_mfma_f<1, 32>(n_station, spectra, ys, nf, spec_tile, frames, integ, s);

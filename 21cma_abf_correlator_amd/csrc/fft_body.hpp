@@ -451,41 +451,35 @@ CORR_HD CORR_FI void fengine_frame(int j, int f, int st, float2 *lds, Sync &sync
 // fixed offset: the base is scalar arithmetic, so a store costs no vector
 // address arithmetic. The mirror bin N - j - q*N/16 is addressed as
 // base(N - (q+1)*N/16) + offset(N/16 - j).
-template <int N>
-struct Fp32Emit {
-    float2 *dst;  // the frame's first bin
-    int j;
-    CORR_HD CORR_FI void begin() {}
-    CORR_HD CORR_FI void at(int k, float2 x) const { dst[(unsigned)k] = x; }
-    CORR_HD CORR_FI void pair(int q, float2 xk, float2 xm) const {
-        constexpr int NT = N / 16;
-        (dst + q * NT)[(unsigned)j] = xk;
-        (dst + (N - (q + 1) * NT))[(unsigned)(NT - j)] = xm;
-    }
-    CORR_HD CORR_FI void zero_at(int k) const { at(k, make_float2(0.0f, 0.0f)); }
-    CORR_HD CORR_FI void zero_pair(int q) const { pair(q, make_float2(0.0f, 0.0f), make_float2(0.0f, 0.0f)); }
-};
+template <int LOG2N, int FORM, class Sync>
+CORR_HD CORR_FI void fengine_form_block(int j, int f, int st, float2 *lds, Sync &sync, const int8_t *in,
+                                       size_t station_pitch, const int32_t *valid, size_t valid_stride,
+                                       const float2 *tw, const float2 *wl, const float *gain,
+                                       void *out, size_t out_station_stride, int tile_shift);
 
-// Whole F-engine workgroup, complex64 output [station][frame][N].
+// Whole F-engine workgroup, complex64 output: [station][frame][N] (tile_shift =
+// LOG2N, the plain order), or tiles of 2^tile_shift channels in the order of
+// forms::spec_index, out_station_stride = N * frame capacity.
 template <int LOG2N, class Sync>
 CORR_HD CORR_FI void fengine_fp32_block(int j, int f, int st, float2 *lds, Sync &sync, const int8_t *in,
                                        size_t station_pitch, const int32_t *valid, size_t valid_stride,
                                        const float2 *tw, const float2 *wl, float2 *out,
-                                       size_t out_station_stride) {
-    constexpr int N = 1 << LOG2N;
-    Fp32Emit<N> e{out + (size_t)st * out_station_stride + (size_t)f * (size_t)N, j};
-    fengine_frame<LOG2N>(j, f, st, lds, sync, in, station_pitch, valid, valid_stride, tw, wl, e);
+                                       size_t out_station_stride, int tile_shift = LOG2N) {
+    fengine_form_block<LOG2N, 0>(j, f, st, lds, sync, in, station_pitch, valid, valid_stride, tw, wl, nullptr, out,
+                                 out_station_stride, tile_shift);
 }
 
 // The form's emitter: bin k scaled by its gain and converted (forms.hpp), at
 // offset off(k) = (k >> shift) * tile_stride + k % 2^shift from the frame's
 // base. off() is additive over a multiple of the tile plus a remainder, which
 // the pair addressing needs: the tile is at most N/16 channels, or N.
+// FORM 0 is the FP32 form: the bin itself as complex64, no gain.
 template <int N, int FORM>
 struct FormEmit {
-    using word_t = typename std::conditional<FORM == 1, uint32_t, uint16_t>::type;
+    using word_t = typename std::conditional<
+        FORM == 0, float2, typename std::conditional<FORM == 1, uint32_t, uint16_t>::type>::type;
     word_t *dst;         // the frame's base in the station's tile-major block
-    const float *g;      // the station's gains
+    const float *g;      // the station's gains (not read in the FP32 form)
     unsigned shift, mask, tile_stride;
     unsigned ok = 0, om = 0;  // off(j), off(N/16 - j), set by begin()
     int j;
@@ -496,25 +490,33 @@ struct FormEmit {
         ok = off((unsigned)j);
         om = off((unsigned)(N / 16 - j));
     }
-    CORR_HD CORR_FI static word_t conv(float2 x, float gain) {
-        if constexpr (FORM == 1)
-            return forms::pack_fp16(x, gain);
+    // bin k's word (gain index k): the gain is read only by the requantizing forms
+    CORR_HD CORR_FI word_t word(float2 x, unsigned k) const {
+        if constexpr (FORM == 0)
+            return x;
+        else if constexpr (FORM == 1)
+            return forms::pack_fp16(x, g[k]);
         else
-            return forms::pack_int8(x, gain);
+            return forms::pack_int8(x, g[k]);
     }
-    CORR_HD CORR_FI void at(int k, float2 x) const { dst[off((unsigned)k)] = conv(x, g[(unsigned)k]); }
+    CORR_HD CORR_FI static word_t zero() {
+        if constexpr (FORM == 0)
+            return make_float2(0.0f, 0.0f);
+        else
+            return (word_t)0;  // zero requantizes to the all-zero word
+    }
+    CORR_HD CORR_FI void at(int k, float2 x) const { dst[off((unsigned)k)] = word(x, (unsigned)k); }
     CORR_HD CORR_FI void pair(int q, float2 xk, float2 xm) const {
         constexpr int NT = N / 16;
         const unsigned bk = (unsigned)(q * NT), bm = (unsigned)(N - (q + 1) * NT);
-        (dst + off(bk))[ok] = conv(xk, (g + bk)[(unsigned)j]);
-        (dst + off(bm))[om] = conv(xm, (g + bm)[(unsigned)(NT - j)]);
+        (dst + off(bk))[ok] = word(xk, bk + (unsigned)j);
+        (dst + off(bm))[om] = word(xm, bm + (unsigned)(NT - j));
     }
-    // zero requantizes to the all-zero word in both forms
-    CORR_HD CORR_FI void zero_at(int k) const { dst[off((unsigned)k)] = 0; }
+    CORR_HD CORR_FI void zero_at(int k) const { dst[off((unsigned)k)] = zero(); }
     CORR_HD CORR_FI void zero_pair(int q) const {
         constexpr int NT = N / 16;
-        (dst + off((unsigned)(q * NT)))[ok] = 0;
-        (dst + off((unsigned)(N - (q + 1) * NT)))[om] = 0;
+        (dst + off((unsigned)(q * NT)))[ok] = zero();
+        (dst + off((unsigned)(N - (q + 1) * NT)))[om] = zero();
     }
 };
 
@@ -523,6 +525,7 @@ struct FormEmit {
 // packed 32-bit (FP16) or 16-bit (int8) complex in the tile-major order of
 // forms::spec_index with tiles of 2^tile_shift channels, out_station_stride =
 // N * frame capacity.
+// FORM 0: the FP32 form's complex64 bins in the same tile-major order (gain unused).
 template <int LOG2N, int FORM, class Sync>
 CORR_HD CORR_FI void fengine_form_block(int j, int f, int st, float2 *lds, Sync &sync, const int8_t *in,
                                        size_t station_pitch, const int32_t *valid, size_t valid_stride,
@@ -534,7 +537,7 @@ CORR_HD CORR_FI void fengine_form_block(int j, int f, int st, float2 *lds, Sync 
     Emit e;
     e.dst = static_cast<typename Emit::word_t *>(out) + (size_t)st * out_station_stride +
             ((size_t)f << tile_shift);
-    e.g = gain + (size_t)st * N;
+    e.g = FORM == 0 ? gain : gain + (size_t)st * N;
     e.shift = (unsigned)tile_shift;
     e.mask = (1u << tile_shift) - 1u;
     e.tile_stride = (unsigned)(out_station_stride / N) << tile_shift;  // fcap * T
@@ -975,19 +978,25 @@ CORR_HD CORR_FI void fengine_mixed_frame(int j, int f, int st, float2 *lds, Sync
     r2c_epilogue_mixed<M, LOG2P>(j, lds, wl, emit);
 }
 
-// Complex64 output [station][frame][N], as fengine_fp32_block.
+template <int M, int LOG2P, int FORM, class Sync>
+CORR_HD CORR_FI void fengine_mixed_form_block(int j, int f, int st, float2 *lds, Sync &sync, const int8_t *in,
+                                             size_t station_pitch, const int32_t *valid, size_t valid_stride,
+                                             const float2 *tw, const float2 *wl, const float *gain, void *out,
+                                             size_t out_station_stride, int tile);
+
+// Complex64 output, as fengine_fp32_block: tile = N is the plain order
+// [station][frame][N], 16 or 32 the tile-major order of forms::spec_index.
 template <int M, int LOG2P, class Sync>
 CORR_HD CORR_FI void fengine_mixed_fp32_block(int j, int f, int st, float2 *lds, Sync &sync, const int8_t *in,
                                              size_t station_pitch, const int32_t *valid, size_t valid_stride,
                                              const float2 *tw, const float2 *wl, float2 *out,
-                                             size_t out_station_stride) {
-    constexpr int N = M << LOG2P;
-    Fp32Emit<N> e{out + (size_t)st * out_station_stride + (size_t)f * (size_t)N, j};
-    fengine_mixed_frame<M, LOG2P>(j, f, st, lds, sync, in, station_pitch, valid, valid_stride, tw, wl, e);
+                                             size_t out_station_stride, int tile = M << LOG2P) {
+    fengine_mixed_form_block<M, LOG2P, 0>(j, f, st, lds, sync, in, station_pitch, valid, valid_stride, tw, wl,
+                                          nullptr, out, out_station_stride, tile);
 }
 
-// The FP16 / int8 forms, as fengine_form_block, with the spectrum tile given
-// in channels: 16, 32 or N (the plain order). For N the emitter's tile shift
+// The FP32 (FORM 0), FP16 and int8 forms, as fengine_form_block, with the
+// spectrum tile given in channels: 16, 32 or N (the plain order). For N the emitter's tile shift
 // is 15, beyond every bin, so that off(k) = k.
 template <int M, int LOG2P, int FORM, class Sync>
 CORR_HD CORR_FI void fengine_mixed_form_block(int j, int f, int st, float2 *lds, Sync &sync, const int8_t *in,
@@ -1000,7 +1009,7 @@ CORR_HD CORR_FI void fengine_mixed_form_block(int j, int f, int st, float2 *lds,
     const unsigned shift = tile == 16 ? 4u : tile == 32 ? 5u : 15u;
     Emit e;
     e.dst = static_cast<typename Emit::word_t *>(out) + (size_t)st * out_station_stride + (size_t)f * (size_t)tile;
-    e.g = gain + (size_t)st * N;
+    e.g = FORM == 0 ? gain : gain + (size_t)st * N;
     e.shift = shift;
     e.mask = (1u << shift) - 1u;
     e.tile_stride = tile == N ? 0u : (unsigned)(out_station_stride / N) * (unsigned)tile;  // fcap * T

@@ -1,7 +1,9 @@
-// xmfma_body.hpp -- the MFMA X-engine of the FP16 and int8 forms, written once for
-// the gfx950 kernel and for a CPU build of the same code (one host thread per
-// lane; the matrix instruction and the lane exchange are provided by an `Ops`
-// object, so the CPU build can emulate them with the same lane layout).
+// xmfma_body.hpp -- the MFMA X-engine of all three forms (FP32 on
+// v_mfma_f32_16x16x4_f32, FP16 on v_mfma_f32_16x16x32_f16, int8 on
+// v_mfma_i32_16x16x64_i8), written once for the gfx950 kernel and for a CPU
+// build of the same code (one host thread per lane; the matrix instruction and
+// the lane exchange are provided by an `Ops` object, so the CPU build can
+// emulate them with the same lane layout).
 //
 // Per channel c the chunk's spectra form a real matrix G (rows x frames):
 //   row 2s   = Re Y_s[t][c],  row 2s+1 = Im Y_s[t][c]   (s = input)
@@ -16,12 +18,17 @@
 // the largest finite half, never as inf; the chunk's sum is added into the integrator once per call,
 // with Neumaier-compensated FP32 (FP16 form) or exactly in int32 (int8 form).
 //
+// The FP32 form (the drop-in's numerics, reference kernels.cu:194-265) takes the
+// same path with complex64 spectra: FP32 products and FP32 sums on the matrix
+// cores, a rounding order of their own within FP32 accuracy.
+//
 // Workgroup: CH = 16 or 32 channels, CH/2 waves, 2 channels per wave. One step
-// stages 32 (FP16) or 64 (int8) frames of all inputs for the CH channels in LDS as
+// stages 16 (FP32), 32 (FP16) or 64 (int8) frames of all inputs for the CH channels in LDS as
 //   lds[channel][row][frame]  (row pitch 80 B = 64 data + 16 pad,
 //                              channel pitch rows_padded*80 + 16 B)
-// so that a lane's MFMA operand (16 B: 8 halves / 16 int8 of one row) is one
-// 16-byte LDS read. The staging reads 16 B (4 channels, FP16) or 8 B (4
+// so that a lane's MFMA operand (16 B: 4 floats / 8 halves / 16 int8 of one
+// row; the 4 floats feed four 16x16x4 products) is one 16-byte LDS read. The
+// staging reads 32 B (4 channels, FP32), 16 B (4 channels, FP16) or 8 B (4
 // channels, int8) per (input, frame) from the [input][frame][channel] spectra
 // and transposes them in registers.
 //
@@ -408,14 +415,28 @@ CORR_HD inline void xengine_mfma_block(int tid, int bx, int n_tiles, uint8_t *ld
                 fr[a].w[2] = v.z;
                 fr[a].w[3] = v.w;
             }
-            int p = 0;
+            if constexpr (FORM == 0) {
+                // FP32: word i of a lane's fragment is frame 4*slot + i of its
+                // row, one K element of the i-th of four 16x16x4 products into
+                // the same accumulator; together they cover the step's 16 frames
+                int p = 0;
 #pragma unroll
-            for (int a = 0; a < NRT; ++a)
+                for (int a = 0; a < NRT; ++a)
 #pragma unroll
-                for (int b = 0; b <= a; ++b) {
-                    acc[ch][p] = ops.mma(fr[a], fr[b], acc[ch][p]);
-                    ++p;
-                }
+                    for (int b = 0; b <= a; ++b) {
+                        acc[ch][p] = ops.mma_f32x4(fr[a], fr[b], acc[ch][p]);
+                        ++p;
+                    }
+            } else {
+                int p = 0;
+#pragma unroll
+                for (int a = 0; a < NRT; ++a)
+#pragma unroll
+                    for (int b = 0; b <= a; ++b) {
+                        acc[ch][p] = ops.mma(fr[a], fr[b], acc[ch][p]);
+                        ++p;
+                    }
+            }
         }
     }
 
@@ -458,7 +479,15 @@ CORR_HD inline void xengine_mfma_block(int tid, int bx, int n_tiles, uint8_t *ld
         const size_t at = (size_t)bl * (size_t)nf + (size_t)(c0 + cl);
         const size_t o = 2 * ((size_t)bl * kOutPitch + cl);
         const acc_t re = vout[o], im = vout[o + 1];
-        if constexpr (FORM == 1) {
+        if constexpr (FORM == 0) {
+            // the call's sum added once into the complex64 row (reference
+            // kernels.cu:216-264 flushes once per chunk the same way)
+            float2 *g = static_cast<float2 *>(integ) + at;
+            float2 cur = *g;
+            cur.x += re;
+            cur.y += im;
+            *g = cur;
+        } else if constexpr (FORM == 1) {
             float4 *g = static_cast<float4 *>(integ) + at;
             float4 cur = *g;
             neumaier(cur.x, cur.z, re);

@@ -4,9 +4,9 @@
 Workload (one "step" = one integration, inputs already resident in HBM):
 24 int8 inputs, n_freq = 16384 (32768-sample real FFTs), 2048 spectra per
 integration: F-engine (mask + unpack + R2C FFT + the form's requantization) and
-X-engine over every chunk of frames (2048 frames per launch in the FP16 and int8
-forms, 512 in the FP32 form: the fastest measured for each), then the dump of
-the integration into complex64 visibilities. The step is one corrx_integrate
+X-engine over every chunk of frames (2048 frames per launch, the fastest
+measured for every form), then the dump of the integration into complex64
+visibilities. The step is one corrx_integrate
 call and the dump; the plain per-chunk loop is timed beside it and must give
 the same bytes. Synthetic correlated-Gaussian int8 data (sigma 16,
 rho 0.1, delays in [0, 32]): 64 distinct frames per input generated with numpy,
@@ -58,8 +58,8 @@ def parse():
     p.add_argument("--n-freq", type=int, default=16384)
     p.add_argument("--frames", type=int, default=2048, help="spectra per integration")
     p.add_argument("--chunk", type=int, default=0,
-                   help="frames per engine launch (0: 2048 for the FP16/int8 forms, 512 for FP32, "
-                        "at most --frames: the fastest measured for each)")
+                   help="frames per engine launch (0: 2048, at most --frames: the fastest measured "
+                        "for every form)")
     p.add_argument("--rho", type=float, default=0.1)
     p.add_argument("--max-delay", type=int, default=32)
     p.add_argument("--forms", default="fp16,fp32,int8",
@@ -378,7 +378,7 @@ def main():
     assert forms and all(f in FORMS for f in forms), args.forms
 
     def chunk_of(name):
-        c = args.chunk or min(T, 512 if name == "fp32" else 2048)
+        c = args.chunk or min(T, 2048)
         assert T % c == 0, (T, c)
         return c
 
@@ -398,7 +398,7 @@ def main():
         chunk = chunk_of(name)
         n_chunks = T // chunk
         eng = corr.Engine(N, nf, chunk, form)
-        xtile = eng.xtile() if form != corr.FORM_FP32 else None
+        xtile = eng.xtile()
         acc = hip.Buffer(B * nf * eng.integrator_bytes())
         cal_ms = None
         if form != corr.FORM_FP32:

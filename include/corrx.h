@@ -136,15 +136,19 @@ corr_status corrx_calibrate(void *engine, const int8_t *d_in, size_t station_pit
 corr_status corrx_set_gains(void *engine, const float *d_gains, void *stream);
 corr_status corrx_gains(void *engine, const float **d_gains);
 
-/* FP16/INT8 forms: channels per X-engine workgroup (16 or 32; 32 only up to 24
-   inputs) and the channel tile of the spectrum buffer's order (that width, or
-   n_freq for the plain order). The engine picks both at creation by timing
+/* Matrix-core X-engine (every form up to 48 inputs): channels per X-engine
+   workgroup (16 or 32; 32 only up to 24 inputs) and the channel tile of the
+   spectrum buffer's order (FP16/INT8: that width, or n_freq for the plain
+   order; FP32: n_freq only). The engine picks both at creation by timing
    F- and X-engine with every combination that fits its shape (cached per
    process, form, input count, n_freq, launch size and device); set overrides
    the choice (before the next corrx_fengine call), get reports it. The
    environment variable CORRX_XTILE=<channels>,<spectra_tile> (spectra_tile 0 =
    n_freq) pins the choice at creation, without timing, for every engine whose
-   shape it fits. */
+   shape it fits. The FP32 form also has a vector X-engine (FMA on the VALU):
+   channels 0 with spectra_tile n_freq selects it (CORRX_XTILE=0,0 at creation);
+   an FP32 engine of more than 48 inputs always runs it, and set fails there
+   for any other choice. get reports channels 0 for it. */
 corr_status corrx_set_xtile(void *engine, int32_t channels, int32_t spectra_tile);
 corr_status corrx_get_xtile(void *engine, int32_t *channels, int32_t *spectra_tile);
 

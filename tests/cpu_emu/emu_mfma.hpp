@@ -9,6 +9,7 @@
 #pragma once
 
 #include <cstdint>
+#include <cstring>
 
 #include "emu_types.hpp"
 #include "forms.hpp"
@@ -66,6 +67,31 @@ struct EmuMfmaOps {
                     s += i8_at(sh->a[wb + 16 * g + row], e) * i8_at(sh->b[wb + 16 * g + col], e);
             d.v[v] = c.v[v] + s;
         }
+        (*sync)();
+        return d;
+    }
+
+    // Four 16x16x4 FP32 products in turn, the i-th on word i of the lanes'
+    // fragments (one float of A and of B per lane, K slice l/16 of 4)
+    corr::xmfma::Acc<float> mma_f32x4(const corr::xmfma::Frag &a, const corr::xmfma::Frag &b,
+                                      const corr::xmfma::Acc<float> &c) {
+        sh->a[tid] = a;
+        sh->b[tid] = b;
+        (*sync)();
+        const int lane = tid & 63, wb = tid & ~63;
+        corr::xmfma::Acc<float> d = c;
+        for (int i = 0; i < 4; ++i)
+            for (int v = 0; v < 4; ++v) {
+                const int row = 4 * (lane >> 4) + v, col = lane & 15;
+                float s = 0.0f;
+                for (int g = 0; g < 4; ++g) {
+                    float x, y;
+                    std::memcpy(&x, &sh->a[wb + 16 * g + row].w[i], 4);
+                    std::memcpy(&y, &sh->b[wb + 16 * g + col].w[i], 4);
+                    s += x * y;
+                }
+                d.v[v] += s;
+            }
         (*sync)();
         return d;
     }

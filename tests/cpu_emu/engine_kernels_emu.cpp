@@ -158,19 +158,27 @@ hipError_t launch_validity(const uint8_t *mask, size_t mask_pitch, int bit0, int
 
 hipError_t launch_fengine_fp32(int n_freq, const int8_t *in, size_t station_pitch,
                                const int32_t *valid, size_t valid_stride, const float2 *tw,
-                               const float2 *wl, float2 *out, size_t out_station_stride,
+                               const float2 *wl, float2 *out, size_t out_station_stride, int tile,
                                int frames, int n_station, hipStream_t) {
     if (frames <= 0 || n_station <= 0) return hipSuccess;
+    if (tile != 16 && tile != 32 && tile != n_freq) return hipErrorInvalidValue;
     switch (n_freq) {
-#define E(L)  \
-    case 1 << L: \
-        return fengine_t<L>(in, station_pitch, valid, valid_stride, tw, wl, out, out_station_stride, frames, n_station);
+#define E(L)                                                                                                    \
+    case 1 << L:                                                                                                \
+        if (tile == n_freq)                                                                                     \
+            return fengine_t<L>(in, station_pitch, valid, valid_stride, tw, wl, out, out_station_stride, frames, \
+                                n_station);                                                                     \
+        return fengine_form_t<L, 0>(in, station_pitch, valid, valid_stride, tw, wl, nullptr, out,              \
+                                    out_station_stride, tile == 16 ? 4 : 5, frames, n_station);
         E(9) E(10) E(11) E(12) E(13) E(14)
 #undef E
 #define E(M, L)                                                                                                   \
     case M << L:                                                                                                  \
-        return fengine_mixed_t<M, L>(in, station_pitch, valid, valid_stride, tw, wl, out, out_station_stride,   \
-                                         frames, n_station);
+        if (tile == n_freq)                                                                                       \
+            return fengine_mixed_t<M, L>(in, station_pitch, valid, valid_stride, tw, wl, out, out_station_stride, \
+                                         frames, n_station);                                                      \
+        return fengine_mixed_form_t<M, L, 0>(in, station_pitch, valid, valid_stride, tw, wl, nullptr, out,       \
+                                             out_station_stride, tile, frames, n_station);
         E(3, 11) E(5, 11) E(3, 12) E(7, 11)
 #undef E
     default:
@@ -184,7 +192,7 @@ hipError_t launch_fengine_staged(int form, int n_freq, const int8_t *in, size_t 
                                  float2 *ys, float2 *zs, hipStream_t) {
     if (frames <= 0 || n_station <= 0) return hipSuccess;
     const fft::StagedPlan pl = fft::staged_plan(n_freq);
-    if (pl.m == 0 || form < 0 || form > 2 || (form != 0 && tile != 16 && tile != 32 && tile != n_freq) ||
+    if (pl.m == 0 || form < 0 || form > 2 || (tile != 16 && tile != 32 && tile != n_freq) ||
         (size_t)frames * (size_t)n_station > 65535u)
         return hipErrorInvalidValue;
     // stage 1, every lane of the device grid (ceil(P/256) x 256 lanes per (k2, frame, station))
@@ -218,9 +226,11 @@ hipError_t launch_fengine_staged(int form, int n_freq, const int8_t *in, size_t 
             const int frame = frame0 + f;
             for (int k = 0; k < lanes3; ++k) {
                 if (form == 0) {
-                    float2 *o = static_cast<float2 *>(out) + (size_t)st * out_station_stride + (size_t)frame * n_freq;
+                    float2 *o = static_cast<float2 *>(out);
+                    auto at = [&](int b) { return forms::spec_index(st, frame, b, n_freq, fcap, tile); };
                     fft::staged_r2c(k, f, st, frames, pl, frame0, valid, valid_stride, zs, wl,
-                                    [&](int b, float2 x) { o[b] = x; }, [&](int b) { o[b] = make_float2(0.0f, 0.0f); });
+                                    [&](int b, float2 x) { o[at(b)] = x; },
+                                    [&](int b) { o[at(b)] = make_float2(0.0f, 0.0f); });
                 } else {
                     const float *g = gain + (size_t)st * n_freq;
                     auto at = [&](int b) { return forms::spec_index(st, frame, b, n_freq, fcap, tile); };
@@ -333,7 +343,7 @@ hipError_t launch_fill(void *p, size_t n_words, uint32_t v, hipStream_t) {
 }
 
 hipError_t xengine_mfma_prepare(int form, int n_station, int ch) {
-    return (form == 1 || form == 2) && xmfma::tile_fits(n_station, ch) ? hipSuccess : hipErrorInvalidValue;
+    return (form >= 0 && form <= 2) && xmfma::tile_fits(n_station, ch) ? hipSuccess : hipErrorInvalidValue;
 }
 
 hipError_t launch_xengine_mfma(int form, int n_station, int ch, const void *spectra, size_t ys, int nf,
@@ -341,6 +351,7 @@ hipError_t launch_xengine_mfma(int form, int n_station, int ch, const void *spec
     if (frames <= 0) return hipSuccess;
     if (!xmfma::tile_fits(n_station, ch) || nf % ch != 0) return hipErrorInvalidValue;
     if (spec_tile != ch && spec_tile != nf) return hipErrorInvalidValue;
+    if (form == 0) return xmfma_f<0>(n_station, ch, spectra, ys, nf, spec_tile, frames, integ);
     if (form == 1) return xmfma_f<1>(n_station, ch, spectra, ys, nf, spec_tile, frames, integ);
     if (form == 2) return xmfma_f<2>(n_station, ch, spectra, ys, nf, spec_tile, frames, integ);
     return hipErrorInvalidValue;

@@ -130,8 +130,91 @@ static int check_fp16(int nst, int nf, int frames, int cap, int ch = 16, bool pl
     return bad;
 }
 
+// FP32 form: complex64 spectra in the plain order (the only order its F-engine
+// writes), against FP64 within FP32 rounding, integrator start values included.
+static int check_fp32(int nst, int nf, int frames, int cap, int ch = 16) {
+    const size_t ys = (size_t)cap * nf;
+    std::mt19937 rng(nst * 13 + frames);
+    std::normal_distribution<float> d(0.0f, 3000.0f);
+    std::vector<float2> y(ys * nst);
+    for (auto &v : y) v = make_float2(d(rng), d(rng));
+    const int B = nst * (nst + 1) / 2;
+    std::vector<float2> acc((size_t)B * nf);
+    for (size_t i = 0; i < acc.size(); ++i) acc[i] = make_float2(1000.0f, -500.0f);
+    dispatch<0>(row_tiles(nst), ch, nf, y.data(), ys, nf, frames, nst, acc.data());
+    double worst = 0;
+    int bad = 0, b = 0;
+    for (int j = 0; j < nst; ++j)
+        for (int k = j; k < nst; ++k, ++b)
+            for (int c = 0; c < nf; ++c) {
+                double sr = 1000.0, si = -500.0, mag = 1.0;
+                for (int t = 0; t < frames; ++t) {
+                    const float2 p = y[(size_t)j * ys + (size_t)t * nf + c];
+                    const float2 q = y[(size_t)k * ys + (size_t)t * nf + c];
+                    sr += (double)p.x * q.x + (double)p.y * q.y;
+                    si += (double)p.y * q.x - (double)p.x * q.y;
+                    mag += std::hypot((double)p.x, (double)p.y) * std::hypot((double)q.x, (double)q.y);
+                }
+                const float2 g = acc[(size_t)b * nf + c];
+                const double e = std::hypot((double)g.x - sr, (double)g.y - si) / mag;
+                if (e > worst) worst = e;
+                if (!(e < 2e-6)) ++bad;
+            }
+    std::printf("fp32 N=%d nf=%d frames=%d ch=%d: worst %.3e, %d bad\n", nst, nf, frames, ch, worst, bad);
+    std::fflush(stdout);
+    return bad;
+}
+
+// FP16 words of +-inf (bins past the half range) are read as +-65504.
+static int check_fp16_saturation() {
+    const int nst = 3, nf = 48, frames = 9, cap = 9;
+    const size_t ys = (size_t)cap * nf;
+    std::vector<uint32_t> y(ys * nst);
+    std::vector<float2> v(ys * nst);
+    std::mt19937 rng(5);
+    std::normal_distribution<float> d(0.0f, 100.0f);
+    for (size_t i = 0; i < y.size(); ++i) {
+        float2 x = make_float2(d(rng), d(rng));
+        if (i % 7 == 3) x.x = 2.0e5f;    // rounds to +inf in FP16
+        if (i % 11 == 5) x.y = -9.0e4f;  // -inf
+        y[i] = corr::forms::pack_fp16(x, 1.0f);
+        const float sr = x.x > 65504.0f ? 65504.0f : x.x, si = x.y < -65504.0f ? -65504.0f : x.y;
+        v[i] = make_float2(corr::forms::f16_bits_to_f32(corr::forms::f32_to_f16_bits(sr)),
+                           corr::forms::f16_bits_to_f32(corr::forms::f32_to_f16_bits(si)));
+    }
+    const int B = nst * (nst + 1) / 2;
+    std::vector<float4> acc((size_t)B * nf, float4{0.0f, 0.0f, 0.0f, 0.0f});
+    dispatch<1>(row_tiles(nst), 16, 16, y.data(), ys, nf, frames, nst, acc.data());
+    int bad = 0, b = 0;
+    for (int j = 0; j < nst; ++j)
+        for (int k = j; k < nst; ++k, ++b)
+            for (int c = 0; c < nf; ++c) {
+                double sr = 0, si = 0, mag = 1.0;
+                for (int t = 0; t < frames; ++t) {
+                    const float2 p = v[corr::forms::spec_index(j, t, c, nf, cap, 16)];
+                    const float2 q = v[corr::forms::spec_index(k, t, c, nf, cap, 16)];
+                    sr += (double)p.x * q.x + (double)p.y * q.y;
+                    si += (double)p.y * q.x - (double)p.x * q.y;
+                    mag += std::hypot((double)p.x, (double)p.y) * std::hypot((double)q.x, (double)q.y);
+                }
+                const float4 g = acc[(size_t)b * nf + c];
+                const double e = std::hypot((double)g.x + g.z - sr, (double)g.y + g.w - si) / mag;
+                if (!(e < 2e-6)) ++bad;  // also counts inf and NaN
+            }
+    std::printf("fp16 saturation: %d bad\n", bad);
+    std::fflush(stdout);
+    return bad;
+}
+
 int main() {
     int bad = 0;
+    bad += check_fp32(1, 48, 3, 4);
+    bad += check_fp32(5, 256, 37, 40);
+    bad += check_fp32(24, 128, 20, 20);
+    bad += check_fp32(24, 96, 33, 40, 32);
+    bad += check_fp32(40, 48, 17, 17);  // five row tiles, two items per lane, prefetched
+    bad += check_fp32(48, 48, 16, 16);  // six row tiles, items in turn
+    bad += check_fp16_saturation();
     // nf 256: 16 channel tiles (XCD mapping permutes); nf 128: 8, nf 48: 3 (identity)
     bad += check_int8(1, 48, 3, 4);
     bad += check_int8(4, 256, 70, 72);

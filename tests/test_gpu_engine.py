@@ -15,7 +15,7 @@ import importlib
 import numpy as np
 import pytest
 
-from conftest import rel_rms
+from conftest import fp32_spectra, rel_rms, to_tiles
 
 pytestmark = pytest.mark.gpu
 
@@ -27,11 +27,7 @@ def hip(corr):
 
 def _spectra(hip, eng, N, T, nf):
     """The engine's complex64 spectrum buffer on the host: (N, T, nf)."""
-    ptr, stride = eng.spectra()
-    out = np.empty((N, stride), dtype=np.complex64)
-    hip.synchronize()
-    hip.check(hip.rt().hipMemcpy(out.ctypes.data, ptr, out.nbytes, hip.D2H), "hipMemcpy")
-    return out[:, : T * nf].reshape(N, T, nf)
+    return fp32_spectra(hip, eng, N, T, nf)
 
 
 # powers of two, the multiples of 2048 with an odd factor (3, 5, 3, 7), and
@@ -87,3 +83,27 @@ def test_c1_fx_through_engine(corr, oracle, hip):
     ref32, _ = oracle.fx_f32(x, chunk_frames=chunk)
     assert rel_rms(got, ref32) < 1e-5
     eng.close()
+
+
+@pytest.mark.parametrize("nf", [2048, 16384, 6144, 32768])
+def test_fp32_spectrum_orders_agree(corr, oracle, hip, nf):
+    """The FP32 F-engine writes the same complex64 bins in every spectrum order
+    the X-engine can read: the plain [station][frame][channel] order and tiles
+    of 16 and 32 channels (forms::spec_index), byte for byte, with a missing
+    frame; in-LDS, odd-factor and staged lengths."""
+    N, T = 3, 5
+    L = 2 * nf
+    x = oracle.gen_int8(12, N, T, L, 0.3, sigma=16.0, max_delay=4)
+    d_in = hip.Buffer.from_array(x)
+    d_mask = hip.Buffer.from_array(np.array([[0xFF], [0xFD], [0xFF]], dtype=np.uint8))
+    got = {}
+    for tile in (nf, 16, 32):
+        print(f"FP32 F-engine nf={nf} spectrum tile {tile}", flush=True)
+        eng = corr.Engine(N, nf, T)
+        eng.set_xtile(16 if tile == nf else tile, tile)
+        eng.fengine(d_in.ptr, T * L, T, d_mask.ptr, 1, 0)
+        got[tile] = fp32_spectra(hip, eng, N, T, nf)
+        eng.close()
+    assert np.all(got[nf][1, 1] == 0)
+    for tile in (16, 32):
+        assert np.array_equal(got[tile].view(np.uint32), got[nf].view(np.uint32)), tile

@@ -18,7 +18,7 @@ import importlib
 import numpy as np
 import pytest
 
-from conftest import rel_rms
+from conftest import fp32_spectra, from_tiles, rel_rms, to_tiles
 
 pytestmark = pytest.mark.gpu
 
@@ -97,8 +97,9 @@ def _engine(corr, N, nf, T, form, ch, tile):
     return eng
 
 
-def test_xtile_rejections(corr, hip):
-    eng = corr.Engine(40, 512, 8, corr.FORM_FP16)
+@pytest.mark.parametrize("form", [0, 1])
+def test_xtile_rejections(corr, hip, form):
+    eng = corr.Engine(40, 512, 8, form)
     assert eng.xtile()[0] == 16
     with pytest.raises(corr.CorrError):
         eng.set_xtile(32, 32)  # 32 channels hold at most 24 inputs
@@ -165,6 +166,59 @@ def test_xengine_fp16(corr, hip, N, nf, T, ch, tile):
     print(f"  worst |error| / sum |products| = {worst:.3e}")
     assert worst < 2e-6
     eng.close()
+
+
+# (N, nf, T, channels per workgroup, spectrum tile): 0 = the tuner's choice,
+# tile -1 = n_freq (the plain order). Every row-tile count of the matrix-core
+# FP32 X-engine, both widths, both spectrum orders, ragged frame counts, and 49
+# inputs, where the FP32 form keeps the vector X-engine
+FP32_CASES = [(1, 512, 3, 0, 0), (5, 512, 37, 0, 0), (8, 1024, 64, 0, 0), (13, 512, 17, 32, -1),
+              (24, 512, 100, 0, 0), (24, 16384, 40, 32, -1), (24, 16384, 40, 32, 32), (24, 16384, 40, 16, 16),
+              (30, 512, 33, 16, -1), (40, 512, 31, 0, 0), (44, 2048, 20, 16, 16), (48, 512, 50, 0, 0),
+              (49, 512, 9, 0, 0), (8, 6144, 33, 32, 32), (24, 2048, 37, -1, -1)]
+
+
+@pytest.mark.parametrize("N,nf,T,ch,tile", FP32_CASES)
+def test_xengine_fp32(corr, hip, N, nf, T, ch, tile):
+    """The FP32 form's X-engine alone on complex64 spectra written straight into
+    the engine's spectrum buffer, accumulated onto start values, against an
+    FP64 sum: every product within FP32 rounding of the sum of |products|."""
+    print(f"FP32 X-engine N={N} nf={nf} T={T} ch={ch} tile={tile}", flush=True)
+    rng = np.random.default_rng(N * 91 + T)
+    if N > 48:
+        eng = corr.Engine(N, nf, T, corr.FORM_FP32)
+        assert eng.xtile() == (0, nf), "more than 48 inputs take the vector X-engine"
+        with pytest.raises(corr.CorrError):
+            eng.set_xtile(16, nf)
+    elif ch < 0:  # the vector X-engine chosen
+        eng = corr.Engine(N, nf, T, corr.FORM_FP32)
+        eng.set_xtile(0, nf)
+        assert eng.xtile() == (0, nf)
+    else:
+        eng = _engine(corr, N, nf, T, corr.FORM_FP32, ch, tile)
+    ptr, stride = eng.spectra()
+    Y = (rng.normal(0.0, 3000.0, size=(N, T, nf)) + 1j * rng.normal(0.0, 3000.0, size=(N, T, nf)))
+    Y = Y.astype(np.complex64)
+    y = np.ascontiguousarray(to_tiles(Y, stride // nf, eng.xtile()[1]))
+    hip.check(hip.rt().hipMemcpy(ptr, y.ctypes.data, y.nbytes, hip.H2D), "hipMemcpy")
+    B = N * (N + 1) // 2
+    start = np.zeros((B, nf), dtype=np.complex64) + (1000.0 - 500.0j)
+    acc = hip.Buffer.from_array(start)
+    eng.xengine(acc.ptr, T)
+    hip.synchronize()
+    got = acc.to_numpy(np.complex64, (B, nf)).astype(np.complex128)
+    eng.close()
+    Yd = Y.astype(np.complex128)
+    A = np.abs(Yd)
+    j, k = np.triu_indices(N)
+    worst = 0.0
+    for b0 in range(0, B, 64):
+        jj, kk = j[b0:b0 + 64], k[b0:b0 + 64]
+        v = np.einsum("btc,btc->bc", Yd[jj], np.conj(Yd[kk])) + (1000.0 - 500.0j)
+        mag = np.einsum("btc,btc->bc", A[jj], A[kk]) + 1.0
+        worst = max(worst, float(np.max(np.abs(got[b0:b0 + 64] - v) / mag)))
+    print(f"  worst |error| / sum |products| = {worst:.3e}")
+    assert worst < 2e-6
 
 
 def _pipeline(corr, oracle, hip, form, N, nf, T, chunk, cal_frames, rho, seed, max_delay):
@@ -298,11 +352,7 @@ def test_fengine_form_rounding(corr, oracle, hip, form, nf):
     pitch = T * 2 * nf
     e32 = corr.Engine(N, nf, T, corr.FORM_FP32)
     e32.fengine(d_in.ptr, pitch, T)
-    hip.synchronize()
-    ptr, stride = e32.spectra()
-    y = np.empty((N, stride), dtype=np.complex64)
-    hip.check(hip.rt().hipMemcpy(y.ctypes.data, ptr, y.nbytes, hip.D2H), "hipMemcpy")
-    y = y.reshape(N, T, nf)
+    y = fp32_spectra(hip, e32, N, T, nf)
     e32.close()
     eng = corr.Engine(N, nf, T, form)
     eng.calibrate(d_in.ptr, pitch, T)
