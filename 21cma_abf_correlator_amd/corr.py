@@ -104,6 +104,7 @@ ENGINE_SYMBOLS = (
     "corrx_engine_create", "corrx_engine_destroy", "corrx_fengine", "corrx_xengine",
     "corrx_integrate", "corrx_batch_create", "corrx_batch_destroy", "corrx_batch_run", "corrx_scale", "corrx_spectra", "corrx_integrator_bytes", "corrx_dump", "corrx_calibrate",
     "corrx_set_gains", "corrx_gains", "corrx_set_xtile", "corrx_get_xtile", "corrx_create_packed",
+    "corrx_correlate_noalloc",
 )
 
 _lib = None
@@ -193,8 +194,14 @@ def declare(L):
         "corrx_gains": ([vp, ctypes.POINTER(vp)], st),
         "corrx_set_xtile": ([vp, i32, i32], st),
         "corrx_get_xtile": ([vp, ctypes.POINTER(i32), ctypes.POINTER(i32)], st),
+        "corrx_correlate_noalloc": ([i32, i32, vp, sz, vp, sz, i32, i64, u64, vp, vp], st),
     }
     for name, (args, res) in sig.items():
+        # the CPU build of the library (tests/cpu_emu) has no static device
+        # scratch, hence no corrx_correlate_noalloc; the GPU build exports all
+        fn = getattr(L, name, None)
+        if fn is None and name == "corrx_correlate_noalloc":
+            continue
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = res

@@ -152,6 +152,33 @@ corr_status corrx_gains(void *engine, const float **d_gains);
 corr_status corrx_set_xtile(void *engine, int32_t channels, int32_t spectra_tile);
 corr_status corrx_get_xtile(void *engine, int32_t *channels, int32_t *spectra_tile);
 
+/* One FP32-form integration that allocates nothing: no counterpart in the
+   reference; it is for callers that cannot allocate device memory (code under
+   HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+   The work is corrx_integrate + corrx_dump on an FP32 engine: F-engine,
+   matrix-core X-engine (16 channels per workgroup, spectra in tiles of 16
+   channels) and the row scale, over `frames` frames (arguments as
+   corrx_integrate), into d_out: complex64 [B][n_freq], zeroed by the call,
+   accumulated, then multiplied by 1.0f/(float)dump_frames (as corr_row_end).
+   In place of an engine's buffers it uses a static scratch array of 256 MiB
+   that each GPU the library runs on holds until the process ends: twiddle
+   tables, then runs of K frames of validity flags and spectra, with
+   K = frames if that fits, else floor((256 MiB - 2 * align256(8 * n_freq)
+   - 512) / (n_station * (8 * n_freq + 4))) (n_freq a power of two; 85 frames
+   at 24 inputs and n_freq 16384). The visibilities are byte-identical to an
+   FP32 engine of chunk_frames K with corrx_set_xtile(e, 16, 16).
+   Limits: 1 <= n_station <= 48; n_freq = 2^k in [512, 16384], 6144, 10240,
+   12288 or 14336 (CORR_ERR_NOT_IMPL otherwise); a frame that does not fit the
+   scratch is refused with CORR_ERR_ARG. Every call on one GPU shares that GPU's
+   array and leaves its contents behind: calls that overlap on one GPU (other
+   streams, other threads) race. The first call on a device looks the array's
+   address up (hipGetSymbolAddress) and raises the X-engine's LDS limit; it
+   allocates nothing either. All launches go on `stream`, with no memset node
+   and no host synchronisation, so a call can be captured into a graph. */
+corr_status corrx_correlate_noalloc(int32_t n_station, int32_t n_freq, const int8_t *d_in, size_t station_pitch,
+                                    const uint8_t *d_mask, size_t mask_pitch, int32_t mask_bit0, int64_t frames,
+                                    uint64_t dump_frames, void *d_out, void *stream);
+
 /* d_acc *= 1.0f/(float)frames over n_complex values. */
 corr_status corrx_scale(void *d_acc, size_t n_complex, uint64_t frames, void *stream);
 

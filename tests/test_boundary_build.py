@@ -73,7 +73,7 @@ def test_every_hip_compile_drops_packed_fp32(cmake_build):
     cmds = json.load(open(cmake_build["bld"] / "compile_commands.json"))
     files = sorted(os.path.basename(c["file"]) for c in cmds)
     assert files == sorted(["batch.cpp", "corr_abi.cpp", "corrx_abi.cpp", "engine.cpp",
-                            "engine_kernels.hip"])
+                            "engine_kernels.hip", "noalloc.hip"])
     for c in cmds:
         line = c.get("command") or " ".join(c["arguments"])
         assert "-packed-fp32-ops" in line and "gfx950" in line, c["file"]
