@@ -400,15 +400,29 @@ CORR_HD CORR_FI float2 r2c_bin(const float2 *lds, const float2 *wl, int k) {
 // Bins k and N-k together (1 <= k < N): with E = (Z[k] + conj(Z[N-k]))/2 and
 // O = (Z[k] - conj(Z[N-k]))/(2i), X[k] = E + W^k O and, as W^(N-k) = -conj(W^k),
 // X[N-k] = conj(E - W^k O).
+// The same from a = Z[k], b = Z[N-k] and wh = W^k / 2: the halvings of E ride
+// in the final FMAs and those of O in wh, 12 instructions per pair where the
+// textbook form takes 16. The roundings are the textbook form's: every
+// halving is exact, so each product and sum rounds the same real value.
+CORR_HD CORR_FI void r2c_combine(float2 a, float2 b, float2 wh, float2 &xk, float2 &xm) {
+    const float s = a.x + b.x, d = a.y - b.y;    // 2 E
+    const float orr = a.y + b.y, oi = b.x - a.x;  // 2 O
+    const float tr = __builtin_fmaf(wh.x, orr, -(wh.y * oi));
+    const float ti = __builtin_fmaf(wh.x, oi, wh.y * orr);
+    xk = make_float2(__builtin_fmaf(0.5f, s, tr), __builtin_fmaf(0.5f, d, ti));
+    xm = make_float2(__builtin_fmaf(0.5f, s, -tr), __builtin_fmaf(-0.5f, d, ti));
+}
+
+// W^k / 2 for k = j + q*N/16: wj / 2 for q = 0, else wj times the constant
+// exp(-i pi q/16) / 2.
+CORR_HD CORR_FI float2 half_twiddle(float2 wj, int q) {
+    return q == 0 ? make_float2(0.5f * wj.x, 0.5f * wj.y)
+                  : cmul(wj, make_float2(0.5f * cos32(q), -0.5f * cos32((q + 24) & 31)));
+}
+
 template <int N>
-CORR_HD CORR_FI void r2c_pair(const float2 *lds, float2 w, int k, float2 &xk, float2 &xm) {
-    const float2 a = lds[k];
-    const float2 b = lds[N - k];
-    const float er = 0.5f * (a.x + b.x), ei = 0.5f * (a.y - b.y);
-    const float orr = 0.5f * (a.y + b.y), oi = -0.5f * (a.x - b.x);
-    const float tr = w.x * orr - w.y * oi, ti = w.x * oi + w.y * orr;
-    xk = make_float2(er + tr, ei + ti);
-    xm = make_float2(er - tr, ti - ei);
+CORR_HD CORR_FI void r2c_pair(const float2 *lds, float2 wh, int k, float2 &xk, float2 &xm) {
+    r2c_combine(lds[k], lds[N - k], wh, xk, xm);
 }
 
 // The R2C epilogue of one workgroup: every retained bin k = 0..N-1 exactly once,
@@ -429,9 +443,8 @@ CORR_HD CORR_FI void r2c_epilogue(int j, const float2 *lds, const float2 *wl, Em
             emit.at(0, r2c_bin<N>(lds, wl, 0));
         } else {
             // exp(-i pi q/16) = cos(2 pi q/32) - i sin(2 pi q/32)
-            const float2 w = q == 0 ? wj : cmul(wj, make_float2(cos32(q), -cos32((q + 24) & 31)));
             float2 xk, xm;
-            r2c_pair<N>(lds, w, k, xk, xm);
+            r2c_pair<N>(lds, half_twiddle(wj, q), k, xk, xm);
             emit.pair(q, xk, xm);
         }
     }
@@ -716,14 +729,12 @@ CORR_HD CORR_FI void fengine_half_16384(int j, float2 (&v)[16], float2 *lds, con
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
         const int k = j + q * NT;
-        const float2 w = q == 0 ? wj : cmul(wj, make_float2(cos32(q), -cos32((q + 24) & 31)));
         if (k == 0) {
             emit.at(0, make_float2(a[0].x + a[0].y, 0.0f));
         } else {
-            const float er = 0.5f * (a[q].x + b[q].x), ei = 0.5f * (a[q].y - b[q].y);
-            const float orr = 0.5f * (a[q].y + b[q].y), oi = -0.5f * (a[q].x - b[q].x);
-            const float tr = w.x * orr - w.y * oi, ti = w.x * oi + w.y * orr;
-            emit.pair(q, make_float2(er + tr, ei + ti), make_float2(er - tr, ti - ei));
+            float2 xk, xm;
+            r2c_combine(a[q], b[q], half_twiddle(wj, q), xk, xm);
+            emit.pair(q, xk, xm);
         }
     }
     if (j == 0) {
@@ -948,8 +959,7 @@ CORR_HD CORR_FI void r2c_epilogue_mixed(int j, const float2 *lds, const float2 *
             r2c_from(z0, z0, wl[0], xk, xm);
             emit.at(0, xk);
         } else {
-            const float2 w = q == 0 ? wj : cmul(wj, make_float2(cos32(q), -cos32((q + 24) & 31)));
-            r2c_from(mixed_z<M, LOG2P>(lds, k), mixed_z<M, LOG2P>(lds, N - k), w, xk, xm);
+            r2c_combine(mixed_z<M, LOG2P>(lds, k), mixed_z<M, LOG2P>(lds, N - k), half_twiddle(wj, q), xk, xm);
             emit.pair(q, xk, xm);
         }
     }
