@@ -104,7 +104,7 @@ ENGINE_SYMBOLS = (
     "corrx_engine_create", "corrx_engine_destroy", "corrx_fengine", "corrx_xengine",
     "corrx_integrate", "corrx_batch_create", "corrx_batch_destroy", "corrx_batch_run", "corrx_scale", "corrx_spectra", "corrx_integrator_bytes", "corrx_dump", "corrx_calibrate",
     "corrx_set_gains", "corrx_gains", "corrx_set_xtile", "corrx_get_xtile", "corrx_create_packed",
-    "corrx_correlate_noalloc",
+    "corrx_correlate_noalloc", "corrx_set_xstep", "corrx_get_xstep",
 )
 
 _lib = None
@@ -195,6 +195,8 @@ def declare(L):
         "corrx_set_xtile": ([vp, i32, i32], st),
         "corrx_get_xtile": ([vp, ctypes.POINTER(i32), ctypes.POINTER(i32)], st),
         "corrx_correlate_noalloc": ([i32, i32, vp, sz, vp, sz, i32, i64, u64, vp, vp], st),
+        "corrx_set_xstep": ([vp, i32], st),
+        "corrx_get_xstep": ([vp, ctypes.POINTER(i32)], st),
     }
     for name, (args, res) in sig.items():
         # the CPU build of the library (tests/cpu_emu) has no static device
@@ -482,6 +484,16 @@ class Engine:
 
     def set_xtile(self, channels, spectra_tile):
         check(self._lib.corrx_set_xtile(self.h, channels, spectra_tile), "corrx_set_xtile", self._lib)
+
+    def xstep(self):
+        """Frames staged per step of the matrix-core X-engine (time-K; 0 for
+        the FP32 vector X-engine)."""
+        n = ctypes.c_int32()
+        check(self._lib.corrx_get_xstep(self.h, ctypes.byref(n)), "corrx_get_xstep", self._lib)
+        return n.value
+
+    def set_xstep(self, frames_per_step):
+        check(self._lib.corrx_set_xstep(self.h, frames_per_step), "corrx_set_xstep", self._lib)
 
 
 class Batch:

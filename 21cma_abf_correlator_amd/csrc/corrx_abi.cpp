@@ -111,6 +111,18 @@ extern "C" corr_status corrx_set_xtile(void *engine, int32_t channels, int32_t s
     return as_engine(engine)->set_xtile(channels, spectra_tile);
 }
 
+extern "C" corr_status corrx_set_xstep(void *engine, int32_t frames_per_step) {
+    if (engine == nullptr) return fail(CORR_ERR_ARG, "engine is NULL");
+    return as_engine(engine)->set_xstep(frames_per_step);
+}
+
+extern "C" corr_status corrx_get_xstep(void *engine, int32_t *frames_per_step) {
+    if (engine == nullptr) return fail(CORR_ERR_ARG, "engine is NULL");
+    if (frames_per_step == nullptr) return fail(CORR_ERR_ARG, "output pointer is NULL");
+    *frames_per_step = as_engine(engine)->xstep();
+    return CORR_OK;
+}
+
 extern "C" corr_status corrx_get_xtile(void *engine, int32_t *channels, int32_t *spectra_tile) {
     if (engine == nullptr) return fail(CORR_ERR_ARG, "engine is NULL");
     if (channels == nullptr || spectra_tile == nullptr) return fail(CORR_ERR_ARG, "output pointer is NULL");

@@ -179,10 +179,11 @@ corr_status Engine::init() {
     }
     if (x_mfma_)
         for (int ch : {16, 32})
-            if (xmfma::tile_fits(N, ch) && nf % ch == 0 &&
-                (st = hip_check(xengine_mfma_prepare((int)shape_.form, N, ch),
-                                "hipFuncSetAttribute(xengine_mfma)")) != CORR_OK)
-                return st;
+            for (int kb : {1, 2, 4})
+                if (xmfma::tile_fits(N, ch, kb) && nf % ch == 0 &&
+                    (st = hip_check(xengine_mfma_prepare((int)shape_.form, N, ch, kb),
+                                    "hipFuncSetAttribute(xengine_mfma)")) != CORR_OK)
+                    return st;
 
     if ((st = hip_check(launch_twiddles(d_tw_, d_wl_, nf, nullptr), "twiddle_kernel")) != CORR_OK)
         return st;
@@ -198,47 +199,70 @@ struct TileKey {
                std::tie(o.form, o.n_station, o.n_freq, o.frames, o.device);
     }
 };
+/// One X-engine shape: channels per workgroup, spectrum tile, K blocks per step.
+struct XShape {
+    int ch, tile, kb;
+    bool operator==(const XShape &o) const { return ch == o.ch && tile == o.tile && kb == o.kb; }
+};
 std::mutex g_tile_mu;
-std::map<TileKey, std::pair<int, int>> g_tile_cache;  // (channels per workgroup, spectrum tile)
+std::map<TileKey, XShape> g_tile_cache;
 constexpr int kTuneRounds = 5;
 constexpr int kTuneFrames = 2048;
 }  // namespace
 
-// The tile autotuner. Two choices interact: the channel width of the
-// X-engine's workgroups (16, or 32 up to 24 inputs) and the channel tile of
-// the spectrum order (that width, whose runs the X-engine reads contiguously,
-// or n_freq, the plain order the F-engine writes fastest). Every combination
-// that fits the shape is timed on the engine's own buffers: F-engine over
-// min(chunk_frames, kTuneFrames) frames of zeros and X-engine over them, the
-// launch size itself (at fewer frames the X-engine's fixed cost per launch
-// hides what the spectrum order changes); one warm-up round, then the fastest
-// of kTuneRounds rounds timed one by one, so that one disturbed round does not
-// decide. The fastest candidate wins. The choice is cached per (form, inputs,
-// n_freq, frames timed, device).
+int Engine::frames_per_block() const {
+    switch (shape_.form) {
+    case Form::kFP16: return xmfma::Traits<1>::kFramesPerBlock;
+    case Form::kINT8: return xmfma::Traits<2>::kFramesPerBlock;
+    default: return xmfma::Traits<0>::kFramesPerBlock;
+    }
+}
+
+// The X-engine autotuner. Three choices interact: the channel width of the
+// X-engine's workgroups (16, or 32 up to 24 inputs), the channel tile of the
+// spectrum order (that width, whose runs the X-engine reads contiguously, or
+// n_freq, the plain order the F-engine writes fastest) and the time-K of a
+// staged step (1, 2 or 4 MFMA K extents of frames, where the staging fits the
+// LDS and a lane stages at most two items: deeper steps halve or quarter the
+// barriers per frame and put more bytes in flight per step, at the price of
+// LDS and registers). Every combination that fits the shape is timed on the
+// engine's own buffers: F-engine over min(chunk_frames, kTuneFrames) frames of
+// zeros and X-engine over them, the launch size itself (at fewer frames the
+// X-engine's fixed cost per launch hides what the spectrum order changes); one
+// warm-up round, then the fastest of kTuneRounds rounds timed one by one, so
+// that one disturbed round does not decide. The fastest candidate wins. The
+// choice is cached per (form, inputs, n_freq, frames timed, device).
 corr_status Engine::tune_xtile() {
     const int N = shape_.n_station, nf = shape_.n_freq;
-    std::vector<std::pair<int, int>> cand;
-    for (int ch : {16, 32})
-        if (xmfma::tile_fits(N, ch) && nf % ch == 0) {
-            cand.emplace_back(ch, ch);
-            cand.emplace_back(ch, nf);
-        }
+    std::vector<XShape> cand;
+    for (int kb : {1, 2, 4})
+        for (int ch : {16, 32})
+            if (xmfma::tile_fits(N, ch, kb) && nf % ch == 0) {
+                cand.push_back({ch, ch, kb});
+                cand.push_back({ch, nf, kb});
+            }
     if (cand.empty()) return failf(CORR_ERR_NOT_IMPL, "xengine: no MFMA tile fits %d inputs", N);
-    x_tile_ = cand[0].first;
-    spec_tile_ = cand[0].second;
-    // CORRX_XTILE=<channels>,<spectrum tile> (tile 0: n_freq, the plain order)
-    // pins the choice wherever it fits the shape, with no timing at creation;
-    // where it does not fit, the tuner runs. "0,0" pins the FP32 form's vector
+    x_tile_ = cand[0].ch;
+    spec_tile_ = cand[0].tile;
+    x_kb_ = cand[0].kb;
+    // CORRX_XTILE=<channels>,<spectrum tile>[,<frames per step>] (tile 0:
+    // n_freq, the plain order; frames per step 0 or absent: one K extent) pins
+    // the choice wherever it fits the shape, with no timing at creation; where
+    // it does not fit, the tuner runs. "0,0" pins the FP32 form's vector
     // X-engine.
     if (const char *env = std::getenv("CORRX_XTILE")) {
-        int ch = 0, tl = -1;
-        if (std::sscanf(env, "%d,%d", &ch, &tl) == 2) {
+        int ch = 0, tl = -1, fps = 0;
+        const int got = std::sscanf(env, "%d,%d,%d", &ch, &tl, &fps);
+        if (got >= 2) {
             if (tl == 0) tl = nf;
             if (ch == 0 && shape_.form == Form::kFP32) return set_xtile(0, nf);
+            const int fb = frames_per_block();
+            const int kb = fps <= 0 ? 1 : (fps % fb == 0 ? fps / fb : -1);
             for (const auto &c : cand)
-                if (c.first == ch && c.second == tl) {
-                    x_tile_ = ch;
-                    spec_tile_ = tl;
+                if (c.ch == ch && c.tile == tl && c.kb == kb) {
+                    x_tile_ = c.ch;
+                    spec_tile_ = c.tile;
+                    x_kb_ = c.kb;
                     return CORR_OK;
                 }
         }
@@ -252,8 +276,9 @@ corr_status Engine::tune_xtile() {
         std::lock_guard<std::mutex> lk(g_tile_mu);
         auto it = g_tile_cache.find(key);
         if (it != g_tile_cache.end()) {
-            x_tile_ = it->second.first;
-            spec_tile_ = it->second.second;
+            x_tile_ = it->second.ch;
+            spec_tile_ = it->second.tile;
+            x_kb_ = it->second.kb;
             return CORR_OK;
         }
     }
@@ -266,7 +291,7 @@ corr_status Engine::tune_xtile() {
     hipStream_t s = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     float best_ms = 0.0f;
-    std::pair<int, int> best = cand[0];
+    XShape best = cand[0];
     if ((st = hip_check(hipMalloc(&integ, integ_bytes), "hipMalloc(tuning integrator)")) == CORR_OK &&
         (st = hip_check(hipMalloc(&in, in_bytes), "hipMalloc(tuning input)")) == CORR_OK &&
         (st = hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate")) == CORR_OK &&
@@ -274,12 +299,12 @@ corr_status Engine::tune_xtile() {
         (st = hip_check(hipEventCreate(&e1), "hipEventCreate")) == CORR_OK &&
         (st = hip_check(launch_fill(in, in_bytes / 4, 0u, s), "fill_kernel(input)")) == CORR_OK &&
         (st = hip_check(launch_fill(integ, integ_bytes / 4, 0u, s), "fill_kernel(integrator)")) == CORR_OK) {
-        auto round = [&](const std::pair<int, int> &c) {
+        auto round = [&](const XShape &c) {
             corr_status r = transform((int)shape_.form, static_cast<const int8_t *>(in), 0, nullptr, 0, frames,
-                                      d_spec_, spectra_station_stride(), c.second, s);
+                                      d_spec_, spectra_station_stride(), c.tile, s);
             if (r != CORR_OK) return r;
-            return hip_check(launch_xengine_mfma((int)shape_.form, N, c.first, d_spec_, spectra_station_stride(),
-                                                 nf, c.second, frames, integ, s),
+            return hip_check(launch_xengine_mfma((int)shape_.form, N, c.ch, c.kb, d_spec_, spectra_station_stride(),
+                                                 nf, c.tile, frames, integ, s),
                              "xengine_mfma_kernel");
         };
         for (const auto &c : cand) {
@@ -308,8 +333,9 @@ corr_status Engine::tune_xtile() {
     hipFree(integ);
     hipFree(in);
     if (st != CORR_OK) return st;
-    x_tile_ = best.first;
-    spec_tile_ = best.second;
+    x_tile_ = best.ch;
+    spec_tile_ = best.tile;
+    x_kb_ = best.kb;
     std::lock_guard<std::mutex> lk(g_tile_mu);
     g_tile_cache[key] = best;
     return CORR_OK;
@@ -335,6 +361,22 @@ corr_status Engine::set_xtile(int channels, int spec_tile) {
     x_mfma_ = true;
     x_tile_ = channels;
     spec_tile_ = spec_tile;
+    // a step depth the new width cannot hold falls back to one K extent
+    if (!xmfma::tile_fits(shape_.n_station, channels, x_kb_)) x_kb_ = 1;
+    return CORR_OK;
+}
+
+corr_status Engine::set_xstep(int frames_per_step) {
+    if (!x_mfma_) return fail(CORR_ERR_ARG, "set_xstep: the vector X-engine has no step depth");
+    const int fb = frames_per_block();
+    const int kb = frames_per_step / fb;
+    if (frames_per_step <= 0 || kb * fb != frames_per_step || !xmfma::kblocks_valid(kb))
+        return failf(CORR_ERR_ARG, "set_xstep: frames per step must be %d, %d or %d in this form, not %d", fb,
+                     2 * fb, 4 * fb, frames_per_step);
+    if (!xmfma::tile_fits(shape_.n_station, x_tile_, kb))
+        return failf(CORR_ERR_ARG, "set_xstep: %d frames per step do not fit %d inputs at %d channels per workgroup",
+                     frames_per_step, shape_.n_station, x_tile_);
+    x_kb_ = kb;
     return CORR_OK;
 }
 
@@ -520,7 +562,7 @@ corr_status Engine::xengine_from(const void *spec, void *integ, int frames, hipS
                                              x_npair_, x_waves_, x_groups_, x_frames_lds_,
                                              static_cast<float2 *>(integ), s),
                          "xengine_fp32_kernel");
-    return hip_check(launch_xengine_mfma((int)shape_.form, shape_.n_station, x_tile_, spec,
+    return hip_check(launch_xengine_mfma((int)shape_.form, shape_.n_station, x_tile_, x_kb_, spec,
                                          spectra_station_stride(), shape_.n_freq, spec_tile_, frames, integ, s),
                      "xengine_mfma_kernel");
 }

@@ -140,6 +140,12 @@ class Engine {
     int xtile() const { return x_tile_; }
     int spec_tile() const { return spec_tile_; }
     corr_status set_xtile(int channels, int spec_tile);
+    /// Time-K of the matrix-core X-engine: frames per staged step, 1, 2 or 4
+    /// K extents of the form's MFMA (frames_per_block(): 16 FP32, 32 FP16,
+    /// 64 int8). Chosen by the tuner with the tile; set_xstep overrides it.
+    int xstep() const { return x_mfma_ ? x_kb_ * frames_per_block() : 0; }
+    corr_status set_xstep(int frames_per_step);
+    int frames_per_block() const;
 
     /// acc *= 1.0f/(float)frames over `n` complex values.
     static corr_status scale(float2 *acc, size_t n, uint64_t frames, hipStream_t s);
@@ -183,6 +189,7 @@ class Engine {
     int cal_frames_ = 1;        ///< frames per calibration batch (FP32 spectra in d_spec_)
     bool x_mfma_ = true;        ///< X-engine on the matrix cores (up to 48 inputs)
     int x_tile_ = 16;           ///< channels per MFMA X-engine workgroup
+    int x_kb_ = 1;              ///< MFMA K extents per staged step (time-K)
     int spec_tile_ = 16;        ///< channel tile of the spectrum order (x_tile_ or n_freq)
     corr_status tune_xtile();
 
@@ -245,8 +252,11 @@ hipError_t launch_power(int nf, int frames, int n_station, const float2 *spec,
 hipError_t launch_gains(int form, int nf, int n_station, const float *power, const int32_t *count,
                         float *gain, hipStream_t s);
 hipError_t launch_fill(void *p, size_t n_words, uint32_t v, hipStream_t s);
-hipError_t xengine_mfma_prepare(int form, int n_station, int ch);
-hipError_t launch_xengine_mfma(int form, int n_station, int ch, const void *spectra, size_t ys, int nf,
+/// The matrix-core X-engine of `form` with ch channels per workgroup and kb
+/// MFMA K blocks per staged step (xmfma::tile_fits): prepare raises its LDS
+/// limit, launch adds `frames` frames of the spectra into the integrator.
+hipError_t xengine_mfma_prepare(int form, int n_station, int ch, int kb);
+hipError_t launch_xengine_mfma(int form, int n_station, int ch, int kb, const void *spectra, size_t ys, int nf,
                                int spec_tile, int frames, void *integ, hipStream_t s);
 hipError_t launch_dump(int form, const void *integ, int n_baseline, int nf, const int2 *pairs,
                        const float *gain, uint64_t frames, float2 *out, hipStream_t s);

@@ -273,22 +273,60 @@ __global__ void __launch_bounds__(kScaleBlock) scale_kernel(float2 *acc, size_t 
 }
 
 // ---------------------------------------------------------------------------
-// MFMA X-engine (FP16 / int8 forms)
-// grid (n_freq/CH), block 32*CH; dynamic LDS xmfma::lds_bytes(n_station, CH).
-// Up to 24 inputs (3 row tiles) a CU holds 16 waves of it (two 16-channel or
-// one 32-channel workgroup), which keeps each lane to 128 registers.
+// MFMA X-engine (every form)
+// grid (n_freq/CH), block 32*CH; dynamic LDS xmfma::lds_bytes(n_station, CH, KB).
+// Up to 24 inputs (3 row tiles) with one staged item per lane, a CU holds 16
+// waves of it (two 16-channel or one 32-channel workgroup), which keeps each
+// lane to 128 registers; with two items per lane (NRT * KB > 4) the kernel may
+// use more registers and holds fewer waves.
 // ---------------------------------------------------------------------------
-template <int FORM, int NRT, int CH>
+template <int FORM, int NRT, int CH, int KB>
 __global__ void __launch_bounds__(xmfma::Tile<CH>::kThreads)
-__attribute__((amdgpu_waves_per_eu(NRT <= 3 ? 4 : 1)))
+__attribute__((amdgpu_waves_per_eu(NRT <= 3 && NRT * KB <= 4 ? 4 : 1)))
 xengine_mfma_kernel(const void *__restrict__ spectra, size_t ys, int nf, int spec_tile, int frames,
                     int n_station, int n_tiles, void *__restrict__ integ) {
     extern __shared__ uint4 lds_words[];
     DeviceSync sync;
     MfmaOps ops;
-    xmfma::xengine_mfma_block<FORM, NRT, CH>((int)threadIdx.x, (int)blockIdx.x, n_tiles,
-                                             reinterpret_cast<uint8_t *>(lds_words), sync, ops,
-                                             spectra, ys, nf, spec_tile, frames, n_station, integ);
+    xmfma::xengine_mfma_block<FORM, NRT, CH, KB>((int)threadIdx.x, (int)blockIdx.x, n_tiles,
+                                                 reinterpret_cast<uint8_t *>(lds_words), sync, ops,
+                                                 spectra, ys, nf, spec_tile, frames, n_station, integ);
+}
+
+template <int FORM, int NRT, int CH, int KB>
+const void *xmfma_kernel_of() {
+    if constexpr (xmfma::shape_built<NRT, CH, KB>())
+        return reinterpret_cast<const void *>(&xengine_mfma_kernel<FORM, NRT, CH, KB>);
+    else
+        return nullptr;
+}
+
+template <int FORM, int CH, int KB>
+const void *xmfma_kernel_of(int nrt) {
+    switch (nrt) {
+    case 1: return xmfma_kernel_of<FORM, 1, CH, KB>();
+    case 2: return xmfma_kernel_of<FORM, 2, CH, KB>();
+    case 3: return xmfma_kernel_of<FORM, 3, CH, KB>();
+    case 4: return xmfma_kernel_of<FORM, 4, CH, KB>();
+    case 5: return xmfma_kernel_of<FORM, 5, CH, KB>();
+    case 6: return xmfma_kernel_of<FORM, 6, CH, KB>();
+    default: return nullptr;
+    }
+}
+
+// The X-engine kernel of (form, row tiles, channels, K blocks), or NULL.
+const void *xmfma_kernel(int form, int nrt, int ch, int kb) {
+#define CORR_XMFMA_OF(F, C, K) \
+    if (form == F && ch == C && kb == K) return xmfma_kernel_of<F, C, K>(nrt);
+#define CORR_XMFMA_FORM(F) \
+    CORR_XMFMA_OF(F, 16, 1) CORR_XMFMA_OF(F, 16, 2) CORR_XMFMA_OF(F, 16, 4) \
+    CORR_XMFMA_OF(F, 32, 1) CORR_XMFMA_OF(F, 32, 2) CORR_XMFMA_OF(F, 32, 4)
+    CORR_XMFMA_FORM(0)
+    CORR_XMFMA_FORM(1)
+    CORR_XMFMA_FORM(2)
+#undef CORR_XMFMA_FORM
+#undef CORR_XMFMA_OF
+    return nullptr;
 }
 
 // Calibration power sum and gains; grid (ceil(nf/256), stations), block 256.
@@ -417,57 +455,6 @@ hipError_t launch_fengine_form_f(int n_freq, const int8_t *in, size_t station_pi
     default:
         return hipErrorInvalidValue;
     }
-}
-
-template <int FORM, int CH>
-const void *xengine_mfma_fn(int nrt) {
-    switch (nrt) {
-    case 1: return reinterpret_cast<const void *>(&xengine_mfma_kernel<FORM, 1, CH>);
-    case 2: return reinterpret_cast<const void *>(&xengine_mfma_kernel<FORM, 2, CH>);
-    case 3: return reinterpret_cast<const void *>(&xengine_mfma_kernel<FORM, 3, CH>);
-    default: break;
-    }
-    if constexpr (CH == 16) {
-        switch (nrt) {
-        case 4: return reinterpret_cast<const void *>(&xengine_mfma_kernel<FORM, 4, CH>);
-        case 5: return reinterpret_cast<const void *>(&xengine_mfma_kernel<FORM, 5, CH>);
-        case 6: return reinterpret_cast<const void *>(&xengine_mfma_kernel<FORM, 6, CH>);
-        default: break;
-        }
-    }
-    return nullptr;
-}
-
-template <int FORM, int CH>
-hipError_t launch_xengine_mfma_f(int n_station, const void *spectra, size_t ys, int nf, int spec_tile,
-                                 int frames, void *integ, hipStream_t s) {
-    const int nrt = xmfma::row_tiles(n_station);
-    const int n_tiles = nf / CH;
-    const size_t shm = xmfma::lds_bytes(n_station, CH);
-    const dim3 grid((unsigned)n_tiles), block((unsigned)xmfma::Tile<CH>::kThreads);
-#define CORR_XMFMA_CASE(R)                                                                           \
-    case R:                                                                                          \
-        hipLaunchKernelGGL((xengine_mfma_kernel<FORM, R, CH>), grid, block, shm, s, spectra, ys, nf, \
-                           spec_tile, frames, n_station, n_tiles, integ);                            \
-        return hipGetLastError();
-    switch (nrt) {
-        CORR_XMFMA_CASE(1)
-        CORR_XMFMA_CASE(2)
-        CORR_XMFMA_CASE(3)
-    default:
-        break;
-    }
-    if constexpr (CH == 16) {
-        switch (nrt) {
-            CORR_XMFMA_CASE(4)
-            CORR_XMFMA_CASE(5)
-            CORR_XMFMA_CASE(6)
-        default:
-            break;
-        }
-    }
-#undef CORR_XMFMA_CASE
-    return hipErrorInvalidValue;
 }
 
 template <int LOG2N>
@@ -662,33 +649,27 @@ hipError_t launch_fill(void *p, size_t n_words, uint32_t v, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t xengine_mfma_prepare(int form, int n_station, int ch) {
-    if (!xmfma::tile_fits(n_station, ch)) return hipErrorInvalidValue;
-    const int nrt = xmfma::row_tiles(n_station);
-    const void *fn = nullptr;
-    if (form == 0) fn = ch == 16 ? xengine_mfma_fn<0, 16>(nrt) : xengine_mfma_fn<0, 32>(nrt);
-    if (form == 1) fn = ch == 16 ? xengine_mfma_fn<1, 16>(nrt) : xengine_mfma_fn<1, 32>(nrt);
-    if (form == 2) fn = ch == 16 ? xengine_mfma_fn<2, 16>(nrt) : xengine_mfma_fn<2, 32>(nrt);
+hipError_t xengine_mfma_prepare(int form, int n_station, int ch, int kb) {
+    if (!xmfma::tile_fits(n_station, ch, kb)) return hipErrorInvalidValue;
+    const void *fn = xmfma_kernel(form, xmfma::row_tiles(n_station), ch, kb);
     if (fn == nullptr) return hipErrorInvalidValue;
     return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)xmfma::lds_bytes(n_station, ch));
+                               (int)xmfma::lds_bytes(n_station, ch, kb));
 }
 
-hipError_t launch_xengine_mfma(int form, int n_station, int ch, const void *spectra, size_t ys, int nf,
+hipError_t launch_xengine_mfma(int form, int n_station, int ch, int kb, const void *spectra, size_t ys, int nf,
                                int spec_tile, int frames, void *integ, hipStream_t s) {
     if (frames <= 0) return hipSuccess;
-    if (!xmfma::tile_fits(n_station, ch) || nf % ch != 0) return hipErrorInvalidValue;
+    if (!xmfma::tile_fits(n_station, ch, kb) || nf % ch != 0) return hipErrorInvalidValue;
     if (spec_tile != ch && spec_tile != nf) return hipErrorInvalidValue;
-    if (form == 0)
-        return ch == 16 ? launch_xengine_mfma_f<0, 16>(n_station, spectra, ys, nf, spec_tile, frames, integ, s)
-                        : launch_xengine_mfma_f<0, 32>(n_station, spectra, ys, nf, spec_tile, frames, integ, s);
-    if (form == 1)
-        return ch == 16 ? launch_xengine_mfma_f<1, 16>(n_station, spectra, ys, nf, spec_tile, frames, integ, s)
-                        : launch_xengine_mfma_f<1, 32>(n_station, spectra, ys, nf, spec_tile, frames, integ, s);
-    if (form == 2)
-        return ch == 16 ? launch_xengine_mfma_f<2, 16>(n_station, spectra, ys, nf, spec_tile, frames, integ, s)
-                        : launch_xengine_mfma_f<2, 32>(n_station, spectra, ys, nf, spec_tile, frames, integ, s);
-    return hipErrorInvalidValue;
+    const void *fn = xmfma_kernel(form, xmfma::row_tiles(n_station), ch, kb);
+    if (fn == nullptr) return hipErrorInvalidValue;
+    int n_tiles = nf / ch;
+    const size_t shm = xmfma::lds_bytes(n_station, ch, kb);
+    // the kernel's arguments in order, each by address (hipLaunchKernel)
+    void *args[] = {&spectra, &ys, &nf, &spec_tile, &frames, &n_station, &n_tiles, &integ};
+    const hipError_t e = hipLaunchKernel(fn, dim3((unsigned)n_tiles), dim3((unsigned)(32 * ch)), args, shm, s);
+    return e != hipSuccess ? e : hipGetLastError();
 }
 
 hipError_t launch_dump(int form, const void *integ, int n_baseline, int nf, const int2 *pairs,

@@ -68,7 +68,7 @@ hipError_t scratch_of_current_device(void **out) {
         // the FP32 X-engine at 16 channels, every row-tile count (inputs 8, 16,
         // ..., 48): raises the dynamic LDS limit where a count needs it
         for (int n = 8; n <= kMaxStationMfma && d.err == hipSuccess; n += 8)
-            d.err = xengine_mfma_prepare(0, n, kTile);
+            d.err = xengine_mfma_prepare(0, n, kTile, 1);
     });
     *out = d.ptr;
     return d.err;
@@ -145,7 +145,7 @@ extern "C" corr_status corrx_correlate_noalloc(int32_t n_station, int32_t n_freq
         if ((st = hip_check(corr::launch_fengine_fp32(nf, d_in + (size_t)f0 * frame_bytes, station_pitch, v, (size_t)K,
                                                       tw, wl, spec, spec_stride, corr::kTile, n, N, s),
                             "fengine_fp32_kernel")) != CORR_OK ||
-            (st = hip_check(corr::launch_xengine_mfma(0, N, corr::kTile, spec, spec_stride, nf, corr::kTile, n, d_out, s),
+            (st = hip_check(corr::launch_xengine_mfma(0, N, corr::kTile, 1, spec, spec_stride, nf, corr::kTile, n, d_out, s),
                             "xengine_mfma_kernel")) != CORR_OK)
             return st;
     }

@@ -49,7 +49,11 @@
 namespace corr {
 namespace xmfma {
 
-constexpr int kRowBytes = 80;       ///< 64 bytes of one row per step + 16 pad
+/// Time-K: a staged step holds KB = 1, 2 or 4 K-extents of one MFMA (16 FP32,
+/// 32 FP16 or 64 int8 frames each). One channel's staged row is then 64 * KB
+/// bytes of data plus 16 of padding.
+CORR_HD constexpr int row_bytes(int kb) { return 64 * kb + 16; }
+CORR_HD constexpr bool kblocks_valid(int kb) { return kb == 1 || kb == 2 || kb == 4; }
 constexpr int kMaxRowTiles = 6;     ///< 2*n_station <= 96
 constexpr int kChanPerWave = 2;
 constexpr int kMaxThreads = 1024;
@@ -81,35 +85,37 @@ template <int FORM>
 struct Traits;
 
 template <>
-struct Traits<0> {  // FP32: complex64 spectra, 4 x v_mfma_f32_16x16x4_f32 per 16 frames
+struct Traits<0> {  // FP32: complex64 spectra, 4 x v_mfma_f32_16x16x4_f32 per K block
     using acc_t = float;
-    static constexpr int kFramesPerStep = 16;
+    static constexpr int kFramesPerBlock = 16;
     static constexpr int kElemBytes = 8;
 };
 
 template <>
 struct Traits<1> {  // FP16
     using acc_t = float;
-    static constexpr int kFramesPerStep = 32;
+    static constexpr int kFramesPerBlock = 32;
     static constexpr int kElemBytes = 4;
 };
 
 template <>
 struct Traits<2> {  // int8
     using acc_t = int32_t;
-    static constexpr int kFramesPerStep = 64;
+    static constexpr int kFramesPerBlock = 64;
     static constexpr int kElemBytes = 2;
 };
 
 CORR_HD inline int rows_padded(int n_station) { return ((2 * n_station + 15) / 16) * 16; }
 CORR_HD inline int row_tiles(int n_station) { return rows_padded(n_station) / 16; }
-CORR_HD inline size_t chan_pitch(int n_station) { return (size_t)rows_padded(n_station) * kRowBytes + 16; }
+CORR_HD inline size_t chan_pitch(int n_station, int kb = 1) {
+    return (size_t)rows_padded(n_station) * (size_t)row_bytes(kb) + 16;
+}
 
 /// Dynamic LDS bytes: the staging tile of ch channels, reused for the results
 /// of one call, [baseline][ch + 1] complex (the pitch one more than the
 /// channels, so that one channel's baselines spread over banks).
-CORR_HD inline size_t lds_bytes(int n_station, int ch) {
-    const size_t stage = (size_t)ch * chan_pitch(n_station);
+CORR_HD inline size_t lds_bytes(int n_station, int ch, int kb = 1) {
+    const size_t stage = (size_t)ch * chan_pitch(n_station, kb);
     const size_t nb = (size_t)n_station * (size_t)(n_station + 1) / 2;
     const size_t out = nb * (size_t)(ch + 1) * 8;
     return stage > out ? stage : out;
@@ -118,9 +124,25 @@ CORR_HD inline size_t lds_bytes(int n_station, int ch) {
 /// Largest LDS of one workgroup (gfx950).
 constexpr size_t kLdsLimit = 160 * 1024;
 
-/// A tile of ch channels is available for n_station inputs.
-CORR_HD inline bool tile_fits(int n_station, int ch) {
-    return (ch == 16 || (ch == 32 && row_tiles(n_station) <= 3)) && lds_bytes(n_station, ch) <= kLdsLimit;
+/// A tile of ch channels and kb K blocks per step is available for n_station
+/// inputs: 32 channels hold at most three row tiles (24 inputs), the staging
+/// fits the LDS, and deeper steps (kb 2 or 4) are kept to n_station * kb <= 64,
+/// where a lane stages at most two items (n_station * ch * kb items over
+/// 32 * ch lanes) and so always has the next step's loads in flight.
+CORR_HD inline bool tile_fits(int n_station, int ch, int kb = 1) {
+    return kblocks_valid(kb) && (kb == 1 || n_station * kb <= 64) &&
+           (ch == 16 || (ch == 32 && row_tiles(n_station) <= 3)) && lds_bytes(n_station, ch, kb) <= kLdsLimit;
+}
+
+/// The (row tiles, channels, K blocks) shapes the library builds: 32 channels
+/// up to three row tiles; deeper steps where their staging fits the LDS at the
+/// largest input count of the row tiles and a lane stages at most two items
+/// (NRT * KB <= 8). Every shape tile_fits admits is among them.
+template <int NRT, int CH, int KB>
+constexpr bool shape_built() {
+    if (CH == 32 && NRT > 3) return false;
+    if (KB == 1) return true;
+    return NRT * KB <= 8 && (size_t)CH * ((size_t)16 * NRT * (size_t)row_bytes(KB) + 16) <= kLdsLimit;
 }
 
 /// Workgroup -> channel tile. Consecutive workgroups are spread over the eight
@@ -159,25 +181,28 @@ struct Stage {
     uint32_t w[32];
 };
 
-// Item = (input s, frame group o of 4, channel quad q of CH/4), q fastest, so
-// that a wave's lanes read whole rows.
-template <int CH>
+// Item = (input s, frame group o of 4*KB, channel quad q of CH/4), q fastest,
+// so that a wave's lanes read whole rows. Frame group o is the 16-byte slot o
+// of the staged rows: K block o/4, lane group o%4 of the MFMA operands.
+template <int CH, int KB>
 CORR_HD inline void item_coords(int item, int &q, int &o, int &s) {
     constexpr int QD = CH / 4;
+    constexpr int G = 4 * KB;
     q = item % QD;
-    o = (item / QD) & 3;
-    s = item / (4 * QD);
+    o = (item / QD) % G;
+    s = item / (G * QD);
 }
 
-template <int FORM, int CH>
+template <int FORM, int CH, int KB>
 struct Staging;
 
-template <int CH>
-struct Staging<0, CH> {  // FP32: 4 frames x 32 B (4 complex64 channels)
+template <int CH, int KB>
+struct Staging<0, CH, KB> {  // FP32: 4 frames x 32 B (4 complex64 channels)
+    static constexpr int kRowBytes = row_bytes(KB);
     static CORR_HD inline void load(int item, const void *spectra, int nf, int fcap, int tile, int frames,
                                     int f0, int c0, Stage &st) {
         int q, o, s;
-        item_coords<CH>(item, q, o, s);
+        item_coords<CH, KB>(item, q, o, s);
         // frames f0 + 4o .. +3 of channels c0 + 4q .. +3: rows `tile` elements apart
         const uint32_t *y = static_cast<const uint32_t *>(spectra) +
                             2 * forms::spec_index(s, f0 + 4 * o, c0 + 4 * q, nf, fcap, tile);
@@ -206,7 +231,7 @@ struct Staging<0, CH> {  // FP32: 4 frames x 32 B (4 complex64 channels)
     // imaginary parts: w[8i + 2m] and w[8i + 2m + 1] of frame i, no arithmetic.
     static CORR_HD inline void store(int item, uint8_t *lds, size_t cpitch, const Stage &st) {
         int q, o, s;
-        item_coords<CH>(item, q, o, s);
+        item_coords<CH, KB>(item, q, o, s);
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
             const uint4 re = make_uint4(st.w[2 * m], st.w[8 + 2 * m], st.w[16 + 2 * m], st.w[24 + 2 * m]);
@@ -220,12 +245,13 @@ struct Staging<0, CH> {  // FP32: 4 frames x 32 B (4 complex64 channels)
     }
 };
 
-template <int CH>
-struct Staging<1, CH> {  // FP16: 8 frames x 16 B (4 channels)
+template <int CH, int KB>
+struct Staging<1, CH, KB> {  // FP16: 8 frames x 16 B (4 channels)
+    static constexpr int kRowBytes = row_bytes(KB);
     static CORR_HD inline void load(int item, const void *spectra, int nf, int fcap, int tile, int frames,
                                     int f0, int c0, Stage &st) {
         int q, o, s;
-        item_coords<CH>(item, q, o, s);
+        item_coords<CH, KB>(item, q, o, s);
         // frames f0 + 8o .. +7 of channels c0 + 4q .. +3: rows `tile` elements apart
         const uint32_t *y = static_cast<const uint32_t *>(spectra) +
                             forms::spec_index(s, f0 + 8 * o, c0 + 4 * q, nf, fcap, tile);
@@ -247,7 +273,7 @@ struct Staging<1, CH> {  // FP16: 8 frames x 16 B (4 channels)
 
     static CORR_HD inline void store(int item, uint8_t *lds, size_t cpitch, const Stage &st) {
         int q, o, s;
-        item_coords<CH>(item, q, o, s);
+        item_coords<CH, KB>(item, q, o, s);
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
             const uint32_t *w = st.w + m;  // w[4 i] = frame i's word of channel m
@@ -278,12 +304,13 @@ struct Staging<1, CH> {  // FP16: 8 frames x 16 B (4 channels)
     }
 };
 
-template <int CH>
-struct Staging<2, CH> {  // int8: 16 frames x 8 B (4 channels)
+template <int CH, int KB>
+struct Staging<2, CH, KB> {  // int8: 16 frames x 8 B (4 channels)
+    static constexpr int kRowBytes = row_bytes(KB);
     static CORR_HD inline void load(int item, const void *spectra, int nf, int fcap, int tile, int frames,
                                     int f0, int c0, Stage &st) {
         int q, o, s;
-        item_coords<CH>(item, q, o, s);
+        item_coords<CH, KB>(item, q, o, s);
         // frames f0 + 16o .. +15 of channels c0 + 4q .. +3: rows `tile` elements apart
         const uint16_t *y = static_cast<const uint16_t *>(spectra) +
                             forms::spec_index(s, f0 + 16 * o, c0 + 4 * q, nf, fcap, tile);
@@ -303,7 +330,7 @@ struct Staging<2, CH> {  // int8: 16 frames x 8 B (4 channels)
 
     static CORR_HD inline void store(int item, uint8_t *lds, size_t cpitch, const Stage &st) {
         int q, o, s;
-        item_coords<CH>(item, q, o, s);
+        item_coords<CH, KB>(item, q, o, s);
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
 #pragma unroll
@@ -327,26 +354,29 @@ struct Staging<2, CH> {  // int8: 16 frames x 8 B (4 channels)
     }
 };
 
-/// Whole X-engine workgroup (CH channels) for `frames` frames; NRT = row tiles.
+/// Whole X-engine workgroup (CH channels) for `frames` frames; NRT = row tiles,
+/// KB = MFMA K blocks per staged step (time-K).
 /// `ops` provides mma(Frag, Frag, Acc) and xor1(value) as wave-wide operations;
 /// every lane of every wave calls them the same number of times.
-template <int FORM, int NRT, int CH, class Sync, class Ops>
+template <int FORM, int NRT, int CH, int KB, class Sync, class Ops>
 CORR_HD inline void xengine_mfma_block(int tid, int bx, int n_tiles, uint8_t *lds, Sync &sync, Ops &ops,
                                        const void *spectra, size_t ys, int nf, int spec_tile, int frames,
                                        int n_station, void *integ) {
     using T = Traits<FORM>;
     using acc_t = typename T::acc_t;
-    using St = Staging<FORM, CH>;
+    using St = Staging<FORM, CH, KB>;
     constexpr int kChannels = Tile<CH>::kChannels;
     constexpr int kThreads = Tile<CH>::kThreads;
     constexpr int kOutPitch = Tile<CH>::kOutPitch;
+    constexpr int kRowBytes = row_bytes(KB);
+    constexpr int kFramesPerStep = T::kFramesPerBlock * KB;
     constexpr int NTP = NRT * (NRT + 1) / 2;
     const int lane = tid & 63, w = tid >> 6;
     const int c0 = channel_tile(bx, n_tiles) * kChannels;
     const int rp = NRT * 16;
     const size_t cpitch = (size_t)rp * kRowBytes + 16;
-    const int n_items = n_station * kChannels;  // inputs x 4 frame groups x CH/4 quads
-    const int fcap = (int)(ys / (size_t)nf);    // frame capacity of the spectrum buffer
+    const int n_items = n_station * kChannels * KB;  // inputs x 4*KB frame groups x CH/4 quads
+    const int fcap = (int)(ys / (size_t)nf);         // frame capacity of the spectrum buffer
 
     // Rows 2N .. rp-1 are never staged: zero them once.
     for (int i = tid; i < kChannels * (rp - 2 * n_station) * (kRowBytes / 16); i += kThreads) {
@@ -365,13 +395,14 @@ CORR_HD inline void xengine_mfma_block(int tid, int bx, int n_tiles, uint8_t *ld
 #pragma unroll
             for (int v = 0; v < 4; ++v) acc[ch][p].v[v] = (acc_t)0;
 
-    // Items of this lane: tid, tid + kThreads (the second only for > 32 inputs).
-    // The next step's loads are issued right after this step's stores, into
-    // the same registers, so they are in flight during the MFMA phase. Two
-    // staged items (64 registers) fit beside the accumulators up to five row
-    // tiles (40 inputs, 120 accumulator registers); at six row tiles they do
-    // not, and the items load each step in turn.
-    constexpr int IPT = (CH * 8 * NRT + kThreads - 1) / kThreads;
+    // Items of this lane: tid, tid + kThreads (the second for more than 32
+    // inputs, or for more than 16 / 32 inputs at KB = 2 / 4). The next step's
+    // loads are issued right after this step's stores, into the same
+    // registers, so they are in flight during the MFMA phase. Two staged items
+    // (64 registers) fit beside the accumulators up to five row tiles (40
+    // inputs, 120 accumulator registers); at six row tiles they do not, and the
+    // items load each step in turn.
+    constexpr int IPT = (CH * 8 * NRT * KB + kThreads - 1) / kThreads;
     constexpr bool kAhead = IPT == 1 || (IPT == 2 && NRT <= 5);
     Stage stg[IPT];
     if constexpr (kAhead) {
@@ -381,7 +412,7 @@ CORR_HD inline void xengine_mfma_block(int tid, int bx, int n_tiles, uint8_t *ld
                 St::load(tid + u * kThreads, spectra, nf, fcap, spec_tile, frames, 0, c0, stg[u]);
     }
 
-    for (int f0 = 0; f0 < frames; f0 += T::kFramesPerStep) {
+    for (int f0 = 0; f0 < frames; f0 += kFramesPerStep) {
         sync();
         if constexpr (kAhead) {
 #pragma unroll
@@ -389,8 +420,8 @@ CORR_HD inline void xengine_mfma_block(int tid, int bx, int n_tiles, uint8_t *ld
                 const int item = tid + u * kThreads;
                 if (item < n_items) {
                     St::store(item, lds, cpitch, stg[u]);
-                    if (f0 + T::kFramesPerStep < frames)
-                        St::load(item, spectra, nf, fcap, spec_tile, frames, f0 + T::kFramesPerStep, c0, stg[u]);
+                    if (f0 + kFramesPerStep < frames)
+                        St::load(item, spectra, nf, fcap, spec_tile, frames, f0 + kFramesPerStep, c0, stg[u]);
                 }
             }
         } else {
@@ -402,10 +433,14 @@ CORR_HD inline void xengine_mfma_block(int tid, int bx, int n_tiles, uint8_t *ld
         }
         sync();
 #pragma unroll
-        for (int ch = 0; ch < kChanPerWave; ++ch) {
+        for (int ch = 0; ch < kChanPerWave; ++ch)
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb) {
+            // K block kb: slots 4 kb .. 4 kb + 3 of the channel's rows (the
+            // swizzle permutes slots within a block)
             const int cl = w * kChanPerWave + ch;
             const uint8_t *base = lds + (size_t)cl * cpitch + (size_t)(lane & 15) * kRowBytes +
-                                  16 * ((lane >> 4) ^ slot_swizzle(cl));
+                                  16 * (4 * kb + ((lane >> 4) ^ slot_swizzle(cl)));
             Frag fr[NRT];
 #pragma unroll
             for (int a = 0; a < NRT; ++a) {

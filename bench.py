@@ -218,7 +218,7 @@ def batch_c5(corr, hip, args, fp64):
         acc = hip.Buffer(B * nf * eng.integrator_bytes())
         out = hip.Buffer(B * nf * 8)
         subs.append(dict(N=N, nf=nf, T=T, L=L, B=B, pool=pool, x=x, eng=eng, acc=acc, out=out,
-                         tile=eng.xtile(), chunk=chunk))
+                         tile=eng.xtile(), step=eng.xstep(), chunk=chunk))
     stream.synchronize()
     batch = corr.Batch([s["eng"] for s in subs])
     items = [corr.Batch.item(s["x"].ptr, s["T"] * s["L"], s["T"], s["acc"].ptr, d_out=s["out"].ptr,
@@ -253,6 +253,7 @@ def batch_c5(corr, hip, args, fp64):
     for s in subs:
         d = {"n_station": s["N"], "n_freq": s["nf"], "spectra": s["T"], "frames_per_launch": s["chunk"],
              "channels_per_workgroup": s["tile"][0], "spectra_tile": s["tile"][1],
+             "frames_per_staged_step": s["step"],
              "integration_ms_realtime": round(s["T"] * s["L"] / 800e6 * 1e3, 3)}
         if fp64:
             V = s["out"].to_numpy(np.complex64, (s["B"], s["nf"]))
@@ -399,6 +400,7 @@ def main():
         n_chunks = T // chunk
         eng = corr.Engine(N, nf, chunk, form)
         xtile = eng.xtile()
+        xstep = eng.xstep()
         acc = hip.Buffer(B * nf * eng.integrator_bytes())
         cal_ms = None
         if form != corr.FORM_FP32:
@@ -492,6 +494,7 @@ def main():
         if xtile is not None:
             kern[f"xengine_{name}"]["channels_per_workgroup"] = xtile[0]
             kern[f"xengine_{name}"]["spectra_tile"] = xtile[1]
+            kern[f"xengine_{name}"]["frames_per_staged_step"] = xstep
         return {"chunk": chunk, "ms_per_step": wall / args.steps * 1e3,
                 "ms_per_step_serial": serial_s * 1e3,
                 "value": 2 * nf * T / (wall / args.steps) / 1e6,

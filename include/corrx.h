@@ -152,6 +152,19 @@ corr_status corrx_gains(void *engine, const float **d_gains);
 corr_status corrx_set_xtile(void *engine, int32_t channels, int32_t spectra_tile);
 corr_status corrx_get_xtile(void *engine, int32_t *channels, int32_t *spectra_tile);
 
+/* Matrix-core X-engine: time-K, the frames staged per step, 1, 2 or 4 times the
+   K extent of the form's MFMA (FP32 16, FP16 32, int8 64 frames). Deeper steps
+   need more LDS and registers per workgroup and are available where the
+   staging fits (n_station * (frames per step / K extent) <= 64). The engine's
+   tuner picks it together with the tile (CORRX_XTILE's optional third field,
+   <channels>,<spectra_tile>,<frames_per_step>, pins it); set overrides it (after
+   corrx_set_xtile, which falls back to one K extent where the new width cannot
+   hold the depth), get reports it (0 for the FP32 vector X-engine). Results do
+   not depend on it beyond the order of FP32 additions. No counterpart in the
+   reference (its kStagedFrames, corr-cuda/src/kernels.cu:103-106, is fixed). */
+corr_status corrx_set_xstep(void *engine, int32_t frames_per_step);
+corr_status corrx_get_xstep(void *engine, int32_t *frames_per_step);
+
 /* One FP32-form integration that allocates nothing: no counterpart in the
    reference; it is for callers that cannot allocate device memory (code under
    HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).

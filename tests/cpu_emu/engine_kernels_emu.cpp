@@ -93,40 +93,47 @@ hipError_t fengine_mixed_form_t(const int8_t *in, size_t station_pitch, const in
     return hipSuccess;
 }
 
-template <int FORM, int NRT, int CH>
+template <int FORM, int NRT, int CH, int KB>
 void xmfma_t(int n_station, const void *spectra, size_t ys, int nf, int spec_tile, int frames, void *integ) {
-    const int n_tiles = nf / CH;
-    EmuMfmaShared *sh = new EmuMfmaShared;
-    run_grid(n_tiles, 1, xmfma::Tile<CH>::kThreads, xmfma::lds_bytes(n_station, CH),
-             [=](int t, int bx, int, char *lds, HostSync &sync) {
-                 EmuMfmaOps ops{t, sh, &sync};
-                 xmfma::xengine_mfma_block<FORM, NRT, CH>(t, bx, n_tiles, reinterpret_cast<uint8_t *>(lds), sync,
-                                                          ops, spectra, ys, nf, spec_tile, frames, n_station, integ);
-             });
-    delete sh;
+    if constexpr (xmfma::shape_built<NRT, CH, KB>()) {  // the others are never admitted by tile_fits
+        const int n_tiles = nf / CH;
+        EmuMfmaShared *sh = new EmuMfmaShared;
+        run_grid(n_tiles, 1, xmfma::Tile<CH>::kThreads, xmfma::lds_bytes(n_station, CH, KB),
+                 [=](int t, int bx, int, char *lds, HostSync &sync) {
+                     EmuMfmaOps ops{t, sh, &sync};
+                     xmfma::xengine_mfma_block<FORM, NRT, CH, KB>(t, bx, n_tiles, reinterpret_cast<uint8_t *>(lds),
+                                                                  sync, ops, spectra, ys, nf, spec_tile, frames,
+                                                                  n_station, integ);
+                 });
+        delete sh;
+    }
+}
+
+// The row-tile counts of a (channels, K blocks) shape, as the GPU build has
+// them (tile_fits has been checked by the caller).
+template <int FORM, int CH, int KB>
+hipError_t xmfma_nrt(int nrt, int n_station, const void *spectra, size_t ys, int nf, int tl, int frames,
+                     void *integ) {
+    switch (nrt) {
+    case 1: xmfma_t<FORM, 1, CH, KB>(n_station, spectra, ys, nf, tl, frames, integ); return hipSuccess;
+    case 2: xmfma_t<FORM, 2, CH, KB>(n_station, spectra, ys, nf, tl, frames, integ); return hipSuccess;
+    case 3: xmfma_t<FORM, 3, CH, KB>(n_station, spectra, ys, nf, tl, frames, integ); return hipSuccess;
+    case 4: xmfma_t<FORM, 4, CH, KB>(n_station, spectra, ys, nf, tl, frames, integ); return hipSuccess;
+    case 5: xmfma_t<FORM, 5, CH, KB>(n_station, spectra, ys, nf, tl, frames, integ); return hipSuccess;
+    case 6: xmfma_t<FORM, 6, CH, KB>(n_station, spectra, ys, nf, tl, frames, integ); return hipSuccess;
+    default: return hipErrorInvalidValue;
+    }
 }
 
 template <int FORM>
-hipError_t xmfma_f(int n_station, int ch, const void *spectra, size_t ys, int nf, int tl, int frames, void *integ) {
+hipError_t xmfma_f(int n_station, int ch, int kb, const void *spectra, size_t ys, int nf, int tl, int frames,
+                   void *integ) {
     const int nrt = xmfma::row_tiles(n_station);
-    if (ch == 32) {
-        switch (nrt) {
-        case 1: xmfma_t<FORM, 1, 32>(n_station, spectra, ys, nf, tl, frames, integ); return hipSuccess;
-        case 2: xmfma_t<FORM, 2, 32>(n_station, spectra, ys, nf, tl, frames, integ); return hipSuccess;
-        case 3: xmfma_t<FORM, 3, 32>(n_station, spectra, ys, nf, tl, frames, integ); return hipSuccess;
-        default: return hipErrorInvalidValue;
-        }
-    }
-    switch (nrt) {
-    case 1: xmfma_t<FORM, 1, 16>(n_station, spectra, ys, nf, tl, frames, integ); break;
-    case 2: xmfma_t<FORM, 2, 16>(n_station, spectra, ys, nf, tl, frames, integ); break;
-    case 3: xmfma_t<FORM, 3, 16>(n_station, spectra, ys, nf, tl, frames, integ); break;
-    case 4: xmfma_t<FORM, 4, 16>(n_station, spectra, ys, nf, tl, frames, integ); break;
-    case 5: xmfma_t<FORM, 5, 16>(n_station, spectra, ys, nf, tl, frames, integ); break;
-    case 6: xmfma_t<FORM, 6, 16>(n_station, spectra, ys, nf, tl, frames, integ); break;
-    default: return hipErrorInvalidValue;
-    }
-    return hipSuccess;
+#define E(C, K) \
+    if (ch == C && kb == K) return xmfma_nrt<FORM, C, K>(nrt, n_station, spectra, ys, nf, tl, frames, integ);
+    E(16, 1) E(16, 2) E(16, 4) E(32, 1) E(32, 2) E(32, 4)
+#undef E
+    return hipErrorInvalidValue;
 }
 
 }  // namespace
@@ -342,18 +349,18 @@ hipError_t launch_fill(void *p, size_t n_words, uint32_t v, hipStream_t) {
     return hipSuccess;
 }
 
-hipError_t xengine_mfma_prepare(int form, int n_station, int ch) {
-    return (form >= 0 && form <= 2) && xmfma::tile_fits(n_station, ch) ? hipSuccess : hipErrorInvalidValue;
+hipError_t xengine_mfma_prepare(int form, int n_station, int ch, int kb) {
+    return (form >= 0 && form <= 2) && xmfma::tile_fits(n_station, ch, kb) ? hipSuccess : hipErrorInvalidValue;
 }
 
-hipError_t launch_xengine_mfma(int form, int n_station, int ch, const void *spectra, size_t ys, int nf,
+hipError_t launch_xengine_mfma(int form, int n_station, int ch, int kb, const void *spectra, size_t ys, int nf,
                                int spec_tile, int frames, void *integ, hipStream_t) {
     if (frames <= 0) return hipSuccess;
-    if (!xmfma::tile_fits(n_station, ch) || nf % ch != 0) return hipErrorInvalidValue;
+    if (!xmfma::tile_fits(n_station, ch, kb) || nf % ch != 0) return hipErrorInvalidValue;
     if (spec_tile != ch && spec_tile != nf) return hipErrorInvalidValue;
-    if (form == 0) return xmfma_f<0>(n_station, ch, spectra, ys, nf, spec_tile, frames, integ);
-    if (form == 1) return xmfma_f<1>(n_station, ch, spectra, ys, nf, spec_tile, frames, integ);
-    if (form == 2) return xmfma_f<2>(n_station, ch, spectra, ys, nf, spec_tile, frames, integ);
+    if (form == 0) return xmfma_f<0>(n_station, ch, kb, spectra, ys, nf, spec_tile, frames, integ);
+    if (form == 1) return xmfma_f<1>(n_station, ch, kb, spectra, ys, nf, spec_tile, frames, integ);
+    if (form == 2) return xmfma_f<2>(n_station, ch, kb, spectra, ys, nf, spec_tile, frames, integ);
     return hipErrorInvalidValue;
 }
 

@@ -15,47 +15,60 @@
 
 using namespace corr::xmfma;
 
-template <int FORM, int NRT, int CH>
+template <int FORM, int NRT, int CH, int KB>
 static void run(const void *y, size_t ys, int nf, int tl, int frames, int nst, void *integ) {
-    constexpr int T = Tile<CH>::kThreads;
-    const int n_tiles = nf / CH;
-    const size_t shm = lds_bytes(nst, CH);
-    uint8_t *lds = new uint8_t[shm];  // heap, exact size: ASan sees the bounds
-    EmuMfmaShared *sh = new EmuMfmaShared;
-    std::barrier<> bar(T);
-    std::vector<std::thread> th;
-    for (int t = 0; t < T; ++t)
-        th.emplace_back([=, &bar] {
-            HostSync sync{&bar};
-            EmuMfmaOps ops{t, sh, &sync};
-            for (int bx = 0; bx < n_tiles; ++bx) {
-                xengine_mfma_block<FORM, NRT, CH>(t, bx, n_tiles, lds, sync, ops, y, ys, nf, tl, frames, nst, integ);
-                sync();
-            }
-        });
-    for (auto &t : th) t.join();
-    delete sh;
-    delete[] lds;
+    if constexpr (shape_built<NRT, CH, KB>()) {
+        constexpr int T = Tile<CH>::kThreads;
+        const int n_tiles = nf / CH;
+        const size_t shm = lds_bytes(nst, CH, KB);
+        uint8_t *lds = new uint8_t[shm];  // heap, exact size: ASan sees the bounds
+        EmuMfmaShared *sh = new EmuMfmaShared;
+        std::barrier<> bar(T);
+        std::vector<std::thread> th;
+        for (int t = 0; t < T; ++t)
+            th.emplace_back([=, &bar] {
+                HostSync sync{&bar};
+                EmuMfmaOps ops{t, sh, &sync};
+                for (int bx = 0; bx < n_tiles; ++bx) {
+                    xengine_mfma_block<FORM, NRT, CH, KB>(t, bx, n_tiles, lds, sync, ops, y, ys, nf, tl, frames,
+                                                          nst, integ);
+                    sync();
+                }
+            });
+        for (auto &t : th) t.join();
+        delete sh;
+        delete[] lds;
+    } else {
+        std::printf("shape NRT=%d CH=%d KB=%d is not built\n", NRT, CH, KB);
+        std::abort();
+    }
+}
+
+// K blocks per staged step of the following checks (time-K).
+static int g_kb = 1;
+
+template <int FORM, int CH, int KB>
+static void dispatch_kb(int nrt, int tl, const void *y, size_t ys, int nf, int frames, int nst, void *integ) {
+    switch (nrt) {
+    case 1: run<FORM, 1, CH, KB>(y, ys, nf, tl, frames, nst, integ); break;
+    case 2: run<FORM, 2, CH, KB>(y, ys, nf, tl, frames, nst, integ); break;
+    case 3: run<FORM, 3, CH, KB>(y, ys, nf, tl, frames, nst, integ); break;
+    case 4: run<FORM, 4, CH, KB>(y, ys, nf, tl, frames, nst, integ); break;
+    case 5: run<FORM, 5, CH, KB>(y, ys, nf, tl, frames, nst, integ); break;
+    case 6: run<FORM, 6, CH, KB>(y, ys, nf, tl, frames, nst, integ); break;
+    }
 }
 
 template <int FORM>
 static void dispatch(int nrt, int ch, int tl, const void *y, size_t ys, int nf, int frames, int nst, void *integ) {
-    if (ch == 32) {
-        switch (nrt) {
-        case 1: run<FORM, 1, 32>(y, ys, nf, tl, frames, nst, integ); break;
-        case 2: run<FORM, 2, 32>(y, ys, nf, tl, frames, nst, integ); break;
-        case 3: run<FORM, 3, 32>(y, ys, nf, tl, frames, nst, integ); break;
-        }
-        return;
+    if (!tile_fits(nst, ch, g_kb)) {
+        std::printf("N=%d ch=%d kb=%d does not fit\n", nst, ch, g_kb);
+        std::abort();
     }
-    switch (nrt) {
-    case 1: run<FORM, 1, 16>(y, ys, nf, tl, frames, nst, integ); break;
-    case 2: run<FORM, 2, 16>(y, ys, nf, tl, frames, nst, integ); break;
-    case 3: run<FORM, 3, 16>(y, ys, nf, tl, frames, nst, integ); break;
-    case 4: run<FORM, 4, 16>(y, ys, nf, tl, frames, nst, integ); break;
-    case 5: run<FORM, 5, 16>(y, ys, nf, tl, frames, nst, integ); break;
-    case 6: run<FORM, 6, 16>(y, ys, nf, tl, frames, nst, integ); break;
-    }
+#define E(C, K) \
+    if (ch == C && g_kb == K) return dispatch_kb<FORM, C, K>(nrt, tl, y, ys, nf, frames, nst, integ);
+    E(16, 1) E(16, 2) E(16, 4) E(32, 1) E(32, 2) E(32, 4)
+#undef E
 }
 
 static int check_int8(int nst, int nf, int frames, int cap, int ch = 16, bool plain = false) {
@@ -238,6 +251,22 @@ int main() {
     bad += check_int8(24, 96, 40, 40, 32, true);
     bad += check_fp16(5, 512, 33, 40, 16, true);
     bad += check_fp16(24, 96, 40, 40, 32, true);
+    // deeper staged steps (time-K): two and four K extents of frames, ragged
+    // frame counts, one and two items per lane, both widths and orders
+    g_kb = 2;
+    bad += check_fp32(13, 48, 70, 72);
+    bad += check_fp32(16, 96, 33, 33, 32);
+    bad += check_fp16(5, 256, 150, 150);
+    bad += check_fp16(24, 48, 70, 70);   // two items per lane
+    bad += check_fp16(16, 96, 90, 90, 32, true);
+    bad += check_int8(9, 48, 200, 200);
+    bad += check_int8(30, 48, 140, 140);  // four row tiles, two items per lane
+    g_kb = 4;
+    bad += check_fp32(8, 96, 70, 70, 32);
+    bad += check_fp16(3, 48, 130, 130);
+    bad += check_fp16(16, 48, 140, 140, 16, true);  // two row tiles, two items per lane
+    bad += check_int8(6, 96, 300, 300, 32);
+    g_kb = 1;
     std::printf(bad ? "FAILED\n" : "OK\n");
     return bad ? 1 : 0;
 }

@@ -17,8 +17,8 @@ seconds). Model: the reference's golden harness (corr-cuda/tests/golden.c:41-48,
       replayed as a hipGraph: every replay byte-equal to the engines run one
       after another, each sub-array within 1e-3 of FP64.
 Plus the FP16 form's behaviour past the half range: gains calibrated on noise
-alone, then a tone 48 dB above it; the dump stays finite and the other channels
-keep their accuracy.
+alone, then a tone bin 61 dB above the noise's per-bin RMS (the form's headroom
+is 48-54 dB); the dump stays finite and the other channels keep their accuracy.
 """
 
 import importlib

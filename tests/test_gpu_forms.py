@@ -87,13 +87,18 @@ FP16_CASES = [(1, 512, 1, 0, 0), (5, 512, 33, 0, 0), (8, 1024, 64, 0, 0), (24, 5
               (48, 512, 50, 16, -1), (8, 6144, 33, 32, 32), (24, 12288, 20, 32, -1), (40, 14336, 9, 16, 16)]
 
 
-def _engine(corr, N, nf, T, form, ch, tile):
+def _engine(corr, N, nf, T, form, ch, tile, step=0):
     eng = corr.Engine(N, nf, T, form)
     c, t = eng.xtile()
     assert c in ((16, 32) if N <= 24 else (16,)) and t in (c, nf)
+    fb = {corr.FORM_FP32: 16, corr.FORM_FP16: 32, corr.FORM_INT8: 64}[form]
+    assert eng.xstep() in (fb, 2 * fb, 4 * fb)
     if ch:
         eng.set_xtile(ch, nf if tile < 0 else tile)
         assert eng.xtile() == (ch, nf if tile < 0 else tile)
+    if step:
+        eng.set_xstep(step)
+        assert eng.xstep() == step
     return eng
 
 
@@ -112,9 +117,13 @@ def test_xtile_rejections(corr, hip, form):
 
 @pytest.mark.parametrize("N,nf,T,ch,tile", INT8_CASES)
 def test_xengine_int8_exact(corr, hip, N, nf, T, ch, tile):
-    print(f"int8 MFMA X-engine N={N} nf={nf} T={T} ch={ch} tile={tile}", flush=True)
+    _xengine_int8(corr, hip, N, nf, T, ch, tile)
+
+
+def _xengine_int8(corr, hip, N, nf, T, ch, tile, step=0):
+    print(f"int8 MFMA X-engine N={N} nf={nf} T={T} ch={ch} tile={tile} step={step}", flush=True)
     rng = np.random.default_rng(N * 1000 + T)
-    eng = _engine(corr, N, nf, T, corr.FORM_INT8, ch, tile)
+    eng = _engine(corr, N, nf, T, corr.FORM_INT8, ch, tile, step)
     _, stride = eng.spectra()
     q = rng.integers(-7, 8, size=(N, T, nf, 2)).astype(np.int8)
     q[0, 0, : min(nf, 4)] = [[7, -7], [-7, 7], [7, 7], [-7, -7]][: min(nf, 4)]
@@ -138,9 +147,13 @@ def test_xengine_int8_exact(corr, hip, N, nf, T, ch, tile):
 
 @pytest.mark.parametrize("N,nf,T,ch,tile", FP16_CASES)
 def test_xengine_fp16(corr, hip, N, nf, T, ch, tile):
-    print(f"FP16 MFMA X-engine N={N} nf={nf} T={T} ch={ch} tile={tile}", flush=True)
+    _xengine_fp16(corr, hip, N, nf, T, ch, tile)
+
+
+def _xengine_fp16(corr, hip, N, nf, T, ch, tile, step=0):
+    print(f"FP16 MFMA X-engine N={N} nf={nf} T={T} ch={ch} tile={tile} step={step}", flush=True)
     rng = np.random.default_rng(N * 77 + T)
-    eng = _engine(corr, N, nf, T, corr.FORM_FP16, ch, tile)
+    eng = _engine(corr, N, nf, T, corr.FORM_FP16, ch, tile, step)
     _, stride = eng.spectra()
     h = rng.normal(0.0, 200.0, size=(N, T, nf, 2)).astype(np.float16)
     words = _to_tiles(h.view(np.uint32).reshape(N, T, nf), stride // nf, eng.xtile()[1])
@@ -183,7 +196,11 @@ def test_xengine_fp32(corr, hip, N, nf, T, ch, tile):
     """The FP32 form's X-engine alone on complex64 spectra written straight into
     the engine's spectrum buffer, accumulated onto start values, against an
     FP64 sum: every product within FP32 rounding of the sum of |products|."""
-    print(f"FP32 X-engine N={N} nf={nf} T={T} ch={ch} tile={tile}", flush=True)
+    _xengine_fp32(corr, hip, N, nf, T, ch, tile)
+
+
+def _xengine_fp32(corr, hip, N, nf, T, ch, tile, step=0):
+    print(f"FP32 X-engine N={N} nf={nf} T={T} ch={ch} tile={tile} step={step}", flush=True)
     rng = np.random.default_rng(N * 91 + T)
     if N > 48:
         eng = corr.Engine(N, nf, T, corr.FORM_FP32)
@@ -195,7 +212,7 @@ def test_xengine_fp32(corr, hip, N, nf, T, ch, tile):
         eng.set_xtile(0, nf)
         assert eng.xtile() == (0, nf)
     else:
-        eng = _engine(corr, N, nf, T, corr.FORM_FP32, ch, tile)
+        eng = _engine(corr, N, nf, T, corr.FORM_FP32, ch, tile, step)
     ptr, stride = eng.spectra()
     Y = (rng.normal(0.0, 3000.0, size=(N, T, nf)) + 1j * rng.normal(0.0, 3000.0, size=(N, T, nf)))
     Y = Y.astype(np.complex64)
@@ -219,6 +236,56 @@ def test_xengine_fp32(corr, hip, N, nf, T, ch, tile):
         worst = max(worst, float(np.max(np.abs(got[b0:b0 + 64] - v) / mag)))
     print(f"  worst |error| / sum |products| = {worst:.3e}")
     assert worst < 2e-6
+
+
+# (form, N, nf, T, channels per workgroup, spectrum tile, frames per staged
+# step): two and four K extents of the form's MFMA staged at once (form 0 FP32
+# 16 frames per extent, 1 FP16 32, 2 int8 64), at the largest input counts
+# they admit (inputs x extents <= 64), both widths and both spectrum orders,
+# with frame counts below one step and ragged past it
+STEP_CASES = [(f, *c[:-1], c[-1] * fb) for f, fb in ((0, 16), (1, 32), (2, 64)) for c in (
+    (4, 512, 0, 32, 32, 4), (8, 1024, 0, 32, -1, 2), (8, 512, 0, 32, 32, 4), (16, 1024, 0, 16, -1, 4),
+    (16, 512, 0, 32, 32, 2), (24, 512, 0, 16, 16, 2), (32, 2048, 0, 16, -1, 2), (13, 6144, 0, 16, 16, 4))]
+STEP_CASES = [(f, N, nf, {0: 37, 1: 70, 2: 130}[f] + 3 * i, ch, tile, step)
+              for i, (f, N, nf, _, ch, tile, step) in enumerate(STEP_CASES)]
+STEP_CASES += [(0, 3, 512, 5, 16, 16, 64), (1, 3, 512, 7, 16, -1, 128), (2, 3, 512, 9, 16, 16, 256)]
+
+
+@pytest.mark.parametrize("form,N,nf,T,ch,tile,step", STEP_CASES)
+def test_xengine_time_k(corr, hip, form, N, nf, T, ch, tile, step):
+    """Each forced step depth against the same checks as the one-extent step:
+    exact integers, FP16 and FP32 products within FP32 rounding."""
+    {0: _xengine_fp32, 1: _xengine_fp16, 2: _xengine_int8}[form](corr, hip, N, nf, T, ch, tile, step)
+
+
+@pytest.mark.parametrize("form", [0, 1, 2])
+def test_xstep_rejections(corr, hip, form):
+    fb = (16, 32, 64)[form]
+    eng = corr.Engine(40, 512, 8, form)
+    for bad in (0, -fb, fb // 2, 3 * fb, 8 * fb):
+        with pytest.raises(corr.CorrError):
+            eng.set_xstep(bad)
+    with pytest.raises(corr.CorrError):
+        eng.set_xstep(2 * fb)  # 40 inputs x 2 extents pass 64
+    eng.set_xstep(fb)
+    eng.close()
+    eng = corr.Engine(16, 512, 8, form)
+    eng.set_xtile(16, 16)
+    eng.set_xstep(4 * fb)
+    eng.set_xtile(32, 32)  # four extents at 32 channels pass the LDS: back to one
+    assert eng.xtile() == (32, 32) and eng.xstep() == fb
+    with pytest.raises(corr.CorrError):
+        eng.set_xstep(4 * fb)
+    eng.set_xstep(2 * fb)
+    assert eng.xstep() == 2 * fb
+    eng.close()
+    if form == 0:
+        eng = corr.Engine(8, 512, 8, form)
+        eng.set_xtile(0, 512)  # the vector X-engine has no step depth
+        assert eng.xstep() == 0
+        with pytest.raises(corr.CorrError):
+            eng.set_xstep(fb)
+        eng.close()
 
 
 def _pipeline(corr, oracle, hip, form, N, nf, T, chunk, cal_frames, rho, seed, max_delay):
