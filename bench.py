@@ -19,8 +19,9 @@ visibilities are compared with FP64 visibilities computed here with numpy (the
 64-frame pool tiled 32 times integrates to exactly the pool's own average).
 
 Prints one JSON line: Msamples/s per input (2*n_freq*T / step time), the
-roofline of the dominant kernel (HIP events on the launch stream), the other
-forms' figures, and the CPU baseline (the oracle's FP32 restatement of the
+roofline of the dominant kernel (HIP events on the launch stream; beside it,
+as roofline.profile, the same from the committed rocprofv3 kernel trace of this
+workload), the other forms' figures, and the CPU baseline (the oracle's FP32 restatement of the
 reference path, timed on a bounded sample on this host).
 """
 
@@ -359,6 +360,22 @@ def measured_traffic(N, nf, chunk, form):
     return t
 
 
+def traced_kernels(N, nf, chunk):
+    """Average duration per launch (µs) of each kernel in the committed
+    rocprofv3 kernel trace of this workload (the default bench.py command run
+    under `rocprofv3 --kernel-trace --marker-trace --selected-regions`);
+    empty when there is none for this geometry."""
+    path = os.path.join(ROOT, "profiles", f"fx_{N}in_nf{nf}_trace.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except OSError:
+        return {}, None
+    if t.get("workload") != f"{N} inputs, n_freq {nf}, {chunk} frames per launch":
+        return {}, None
+    return t.get("avg_us", {}), os.path.relpath(path, ROOT)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -547,6 +564,19 @@ def main():
         roof = {"bound": "hbm", "achieved": round(h["x_gbs"], 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(h["x_gbs"] / HBM_PEAK_GBS, 4),
                 "traffic": traffic.get(f"xengine_{head}"), "kernel": f"xengine_{head}"}
+    # the same roofline from the committed kernel trace's average duration of
+    # that kernel, beside the one from this run's HIP events
+    traced, source = traced_kernels(N, nf, h["chunk"])
+    if roof["kernel"] in traced:
+        us = traced[roof["kernel"]]
+        kk = h["kernels"][roof["kernel"]]
+        if roof["unit"] == "GB/s":
+            ach = kk["algorithmic_bytes"] / (us * 1e-6) / 1e9
+        else:
+            ach = kk["algorithmic_flops"] / (us * 1e-6) / 1e12
+        roof["profile"] = {"avg_us": us, "achieved": round(ach, 2 if roof["unit"] == "TFLOP/s" else 1),
+                           "frac": round(ach / roof["peak"], 4), "source": source,
+                           "event_over_trace": round(kk["avg_ms"] * 1e3 / us, 4)}
     value = h["value"]
     ms_per_step = h["ms_per_step"]
     kernels = h["kernels"]
