@@ -79,6 +79,10 @@ corr_status Engine::create(const EngineShape &shape, Engine **out) {
     if (shape.form != Form::kFP32 && shape.n_station > kMaxStationMfma)
         return failf(CORR_ERR_NOT_IMPL, "engine: the FP16/int8 forms take at most %d inputs (got %d)",
                      kMaxStationMfma, shape.n_station);
+    // the F-engine addresses one input's spectra with 32-bit element offsets
+    if ((size_t)shape.chunk_frames * (size_t)shape.n_freq > (size_t)UINT32_MAX)
+        return failf(CORR_ERR_ARG, "engine: chunk_frames=%d x n_freq=%d: one input's spectra must stay under 2^32 values",
+                     shape.chunk_frames, shape.n_freq);
     Engine *e = new (std::nothrow) Engine();
     if (e == nullptr) return fail(CORR_ERR_NOMEM, "engine: out of host memory");
     e->shape_ = shape;

@@ -35,7 +35,8 @@ extern "C" {
 typedef struct corrx_shape {
     int32_t n_station;    /* inputs, >= 1                                    */
     int32_t n_freq;       /* retained channels: 2^k in [512, 16384], or a multiple of 2048 up to 1048576 */
-    int32_t chunk_frames; /* most frames one fengine/xengine call handles    */
+    int32_t chunk_frames; /* most frames one fengine/xengine call handles;
+                             chunk_frames * n_freq < 2^32                 */
     int32_t form;         /* CORRX_FORM_*                                    */
 } corrx_shape;
 
@@ -195,12 +196,12 @@ corr_status corrx_correlate_noalloc(int32_t n_station, int32_t n_freq, const int
 /* d_acc *= 1.0f/(float)frames over n_complex values. */
 corr_status corrx_scale(void *d_acc, size_t n_complex, uint64_t frames, void *stream);
 
-/* The engine's spectrum buffer. FP32: complex64 [station][chunk_frames][n_freq].
-   FP16 and INT8: [station][n_freq/T][chunk_frames][T] with T the spectra tile
-   of corrx_get_xtile (T = n_freq is the plain [station][frame][channel] order),
-   of two IEEE halves in 32 bits, real in the low half (FP16), or two int8 in 16
-   bits, real in the low byte (INT8), each the spectrum times the gain of its
-   (input, channel). station_stride_elems = chunk_frames * n_freq. */
+/* The engine's spectrum buffer, every form: [station][n_freq/T][chunk_frames][T]
+   with T the spectra tile of corrx_get_xtile (T = n_freq is the plain
+   [station][frame][channel] order), of complex64 (FP32), two IEEE halves in 32
+   bits, real in the low half (FP16), or two int8 in 16 bits, real in the low
+   byte (INT8); the FP16 and INT8 values are the spectrum times the gain of
+   their (input, channel). station_stride_elems = chunk_frames * n_freq. */
 corr_status corrx_spectra(void *engine, void **d_spectra, size_t *station_stride_elems);
 
 /* A drop-in context (corr_cuda.h) whose staging holds samples of `sample_bits`

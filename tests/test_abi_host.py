@@ -200,6 +200,7 @@ def test_pack4_layout(corr):
     ((4, 20000, 16, 0), "ERR_NOT_IMPL"),  # above 16384 and not a multiple of 2048
     ((0, 1024, 16, 2), "ERR_ARG"),
     ((4, 1024, 0, 1), "ERR_ARG"),
+    ((1, 16384, 262145, 0), "ERR_ARG"),   # one input's spectra past 2^32 values
 ])
 def test_engine_shape_rejections(corr, shape, status):
     """Shapes the engine refuses before touching a device."""
