@@ -430,12 +430,34 @@ static int check_staged(int n) {
     return bad;
 }
 
-int main() {
-    int bad = check_half();
-    bad += check<9>() + check<10>() + check<11>() + check<12>() + check<13>() + check<14>();
-    bad += check_mixed<3, 11>() + check_mixed<5, 11>() + check_mixed<3, 12>() + check_mixed<7, 11>();
-    bad += check_unpack4();
-    bad += check_staged(18432) + check_staged(32768) + check_staged(45056) + check_staged(67584);
+// `fengine_emu <part> <parts>` runs every parts-th check from the part-th on, so
+// that the checks can run in parallel processes; no arguments run them all.
+static int g_part = 0, g_parts = 1, g_idx = 0;
+static bool mine() { return (g_idx++ % g_parts) == g_part; }
+
+int main(int argc, char **argv) {
+    if (argc == 3) {
+        g_part = std::atoi(argv[1]);
+        g_parts = std::atoi(argv[2]);
+        if (g_parts < 1 || g_part < 0 || g_part >= g_parts) return 2;
+    }
+    int bad = 0;
+    if (mine()) bad += check_half();
+    if (mine()) bad += check<9>();
+    if (mine()) bad += check<10>();
+    if (mine()) bad += check<11>();
+    if (mine()) bad += check<12>();
+    if (mine()) bad += check<13>();
+    if (mine()) bad += check<14>();
+    if (mine()) bad += check_mixed<3, 11>();
+    if (mine()) bad += check_mixed<5, 11>();
+    if (mine()) bad += check_mixed<3, 12>();
+    if (mine()) bad += check_mixed<7, 11>();
+    if (mine()) bad += check_unpack4();
+    if (mine()) bad += check_staged(18432);
+    if (mine()) bad += check_staged(32768);
+    if (mine()) bad += check_staged(45056);
+    if (mine()) bad += check_staged(67584);
     std::printf(bad ? "FAILED\n" : "OK\n");
     return bad ? 1 : 0;
 }

@@ -220,28 +220,57 @@ def timed(fn, *args):
     return r
 
 
-def main():
-    # The stand-in runtime's timer reads zero, so tuning the X-engine tile would
-    # only cost emulation time here: the tile is pinned (the GPU tests tune).
-    # The matrix-core X-engine's 512-lane workgroups exchange operands through
-    # a workgroup barrier per product in this emulation, which makes it slow:
-    # one drop-in run covers the FP32 form on it, the others pin the FP32
-    # form's vector X-engine ("0,0"; the precision forms still take 16,16).
+# (CORRX_XTILE, case, arguments after the library). The stand-in runtime's
+# timer reads zero, so tuning the X-engine tile would only cost emulation time
+# here: the tile is pinned (the GPU tests tune). The matrix-core X-engine's
+# 512-lane workgroups exchange operands through a workgroup barrier per product
+# in this emulation, which makes it slow: one drop-in run covers the FP32 form
+# on it, the others pin the FP32 form's vector X-engine ("0,0"); the small
+# precision-form shapes take the matrix-core X-engine of every form ("16,16").
+CASES = [
+    ("16,16", golden, (3, 2048, 20, 20, 1)),
+    ("0,0", golden, (4, 2048, 90, 32, 2)),
+    ("0,0", golden, (5, 4096, 37, 16, 3)),
+    ("0,0", golden, (3, 6144, 20, 16, 1)),   # n_freq = 3 x 2048
+    ("0,0", golden, (2, 18432, 6, 4, 1)),    # 9 x 2048, staged through HBM
+    ("0,0", packed, (3, 2048, 40, 16, 2)),
+    ("0,0", skip, ()),
+    ("16,16", forms, (corr.FORM_FP16, 5, 512, 70, 32, 20)),
+    ("16,16", forms, (corr.FORM_INT8, 6, 512, 70, 32, 70)),
+    ("16,16", integrate_fp32, (3, 512, 45, 8)),
+    ("16,16", batch, ()),
+]
+
+
+def run_case(i):
+    xtile, fn, args = CASES[i]
+    os.environ["CORRX_XTILE"] = xtile
     L = corr.declare(ctypes.CDLL(os.path.join(HERE, "_build", "libcorr_cuda_emu.so")))
-    os.environ["CORRX_XTILE"] = "16,16"
-    ok = timed(golden, L, 3, 2048, 20, 20, 1)
-    os.environ["CORRX_XTILE"] = "0,0"
-    ok = timed(golden, L, 4, 2048, 90, 32, 2) and ok
-    ok = timed(golden, L, 5, 4096, 37, 16, 3) and ok
-    ok = timed(golden, L, 3, 6144, 20, 16, 1) and ok  # n_freq = 3 x 2048
-    ok = timed(golden, L, 2, 18432, 6, 4, 1) and ok   # 9 x 2048, staged through HBM
-    ok = timed(packed, L, 3, 2048, 40, 16, 2) and ok
-    ok = timed(skip, L) and ok
-    os.environ["CORRX_XTILE"] = "16,16"  # small shapes: the matrix-core X-engine of every form
-    ok = timed(forms, L, corr.FORM_FP16, 5, 512, 70, 32, 20) and ok
-    ok = timed(forms, L, corr.FORM_INT8, 6, 512, 70, 32, 70) and ok
-    ok = timed(integrate_fp32, L, 3, 512, 45, 8) and ok
-    ok = timed(batch, L) and ok
+    return timed(fn, L, *args)
+
+
+def main():
+    """Every case in a process of its own (the cases share nothing), at most
+    eight at a time; each prints its own lines, shown in case order."""
+    if len(sys.argv) == 3 and sys.argv[1] == "--case":
+        return 0 if run_case(int(sys.argv[2])) else 1
+    import subprocess
+    from concurrent.futures import ThreadPoolExecutor
+
+    def child(i):
+        return subprocess.run([sys.executable, os.path.abspath(__file__), "--case", str(i)],
+                              capture_output=True, text=True, timeout=900)
+
+    workers = max(1, min(8, os.cpu_count() or 1))
+    with ThreadPoolExecutor(max_workers=workers) as pool:
+        results = list(pool.map(child, range(len(CASES))))
+    ok = True
+    for i, r in enumerate(results):
+        print(f"case {i}: {CASES[i][1].__name__}{CASES[i][2]} CORRX_XTILE={CASES[i][0]}", flush=True)
+        print(r.stdout, end="", flush=True)
+        if r.returncode != 0:
+            print(r.stderr[-3000:], flush=True)
+            ok = False
     print("OK" if ok else "FAILED")
     return 0 if ok else 1
 

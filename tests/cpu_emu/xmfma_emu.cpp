@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <random>
 #include <thread>
 #include <vector>
@@ -219,53 +220,63 @@ static int check_fp16_saturation() {
     return bad;
 }
 
-int main() {
+// `xmfma_emu <part> <parts>` runs every parts-th check from the part-th on, so
+// that the checks can run in parallel processes; no arguments run them all.
+static int g_part = 0, g_parts = 1, g_idx = 0;
+static bool mine() { return (g_idx++ % g_parts) == g_part; }
+
+int main(int argc, char **argv) {
+    if (argc == 3) {
+        g_part = std::atoi(argv[1]);
+        g_parts = std::atoi(argv[2]);
+        if (g_parts < 1 || g_part < 0 || g_part >= g_parts) return 2;
+    }
     int bad = 0;
-    bad += check_fp32(1, 48, 3, 4);
-    bad += check_fp32(5, 256, 37, 40);
-    bad += check_fp32(24, 128, 20, 20);
-    bad += check_fp32(24, 96, 33, 40, 32);
-    bad += check_fp32(40, 48, 17, 17);  // five row tiles, two items per lane, prefetched
-    bad += check_fp32(48, 48, 16, 16);  // six row tiles, items in turn
-    bad += check_fp16_saturation();
+    if (mine()) bad += check_fp32(1, 48, 3, 4);
+    if (mine()) bad += check_fp32(5, 256, 37, 40);
+    if (mine()) bad += check_fp32(24, 128, 20, 20);
+    if (mine()) bad += check_fp32(24, 96, 33, 40, 32);
+    if (mine()) bad += check_fp32(40, 48, 17, 17);  // five row tiles, two items per lane, prefetched
+    if (mine()) bad += check_fp32(48, 48, 16, 16);  // six row tiles, items in turn
+    if (mine()) bad += check_fp16_saturation();
     // nf 256: 16 channel tiles (XCD mapping permutes); nf 128: 8, nf 48: 3 (identity)
-    bad += check_int8(1, 48, 3, 4);
-    bad += check_int8(4, 256, 70, 72);
-    bad += check_int8(9, 48, 130, 130);
-    bad += check_int8(24, 128, 64, 64);
-    bad += check_int8(30, 48, 20, 20);
-    bad += check_int8(36, 48, 70, 70);  // five row tiles, two items per lane, prefetched
-    bad += check_int8(48, 48, 65, 66);
-    bad += check_fp16(1, 48, 1, 2);
-    bad += check_fp16(5, 256, 33, 40);
-    bad += check_fp16(8, 48, 64, 64);
-    bad += check_fp16(24, 128, 20, 20);
-    bad += check_fp16(40, 48, 70, 70);  // five row tiles, two items per lane, prefetched
+    if (mine()) bad += check_int8(1, 48, 3, 4);
+    if (mine()) bad += check_int8(4, 256, 70, 72);
+    if (mine()) bad += check_int8(9, 48, 130, 130);
+    if (mine()) bad += check_int8(24, 128, 64, 64);
+    if (mine()) bad += check_int8(30, 48, 20, 20);
+    if (mine()) bad += check_int8(36, 48, 70, 70);  // five row tiles, two items per lane, prefetched
+    if (mine()) bad += check_int8(48, 48, 65, 66);
+    if (mine()) bad += check_fp16(1, 48, 1, 2);
+    if (mine()) bad += check_fp16(5, 256, 33, 40);
+    if (mine()) bad += check_fp16(8, 48, 64, 64);
+    if (mine()) bad += check_fp16(24, 128, 20, 20);
+    if (mine()) bad += check_fp16(40, 48, 70, 70);  // five row tiles, two items per lane, prefetched
     // 32 channels per workgroup (up to 24 inputs): nf 512 = 16 tiles, nf 96 = 3
-    bad += check_int8(3, 96, 70, 70, 32);
-    bad += check_int8(24, 96, 40, 40, 32);
-    bad += check_fp16(13, 96, 33, 33, 32);
-    bad += check_fp16(8, 512, 20, 20, 32);
+    if (mine()) bad += check_int8(3, 96, 70, 70, 32);
+    if (mine()) bad += check_int8(24, 96, 40, 40, 32);
+    if (mine()) bad += check_fp16(13, 96, 33, 33, 32);
+    if (mine()) bad += check_fp16(8, 512, 20, 20, 32);
     // the plain [input][frame][channel] order (a spectrum tile of n_freq)
-    bad += check_int8(9, 96, 70, 72, 16, true);
-    bad += check_int8(24, 96, 40, 40, 32, true);
-    bad += check_fp16(5, 512, 33, 40, 16, true);
-    bad += check_fp16(24, 96, 40, 40, 32, true);
+    if (mine()) bad += check_int8(9, 96, 70, 72, 16, true);
+    if (mine()) bad += check_int8(24, 96, 40, 40, 32, true);
+    if (mine()) bad += check_fp16(5, 512, 33, 40, 16, true);
+    if (mine()) bad += check_fp16(24, 96, 40, 40, 32, true);
     // deeper staged steps (time-K): two and four K extents of frames, ragged
     // frame counts, one and two items per lane, both widths and orders
     g_kb = 2;
-    bad += check_fp32(13, 48, 70, 72);
-    bad += check_fp32(16, 96, 33, 33, 32);
-    bad += check_fp16(5, 256, 150, 150);
-    bad += check_fp16(24, 48, 70, 70);   // two items per lane
-    bad += check_fp16(16, 96, 90, 90, 32, true);
-    bad += check_int8(9, 48, 200, 200);
-    bad += check_int8(30, 48, 140, 140);  // four row tiles, two items per lane
+    if (mine()) bad += check_fp32(13, 48, 70, 72);
+    if (mine()) bad += check_fp32(16, 96, 33, 33, 32);
+    if (mine()) bad += check_fp16(5, 256, 150, 150);
+    if (mine()) bad += check_fp16(24, 48, 70, 70);   // two items per lane
+    if (mine()) bad += check_fp16(16, 96, 90, 90, 32, true);
+    if (mine()) bad += check_int8(9, 48, 200, 200);
+    if (mine()) bad += check_int8(30, 48, 140, 140);  // four row tiles, two items per lane
     g_kb = 4;
-    bad += check_fp32(8, 96, 70, 70, 32);
-    bad += check_fp16(3, 48, 130, 130);
-    bad += check_fp16(16, 48, 140, 140, 16, true);  // two row tiles, two items per lane
-    bad += check_int8(6, 96, 300, 300, 32);
+    if (mine()) bad += check_fp32(8, 96, 70, 70, 32);
+    if (mine()) bad += check_fp16(3, 48, 130, 130);
+    if (mine()) bad += check_fp16(16, 48, 140, 140, 16, true);  // two row tiles, two items per lane
+    if (mine()) bad += check_int8(6, 96, 300, 300, 32);
     g_kb = 1;
     std::printf(bad ? "FAILED\n" : "OK\n");
     return bad ? 1 : 0;

@@ -15,17 +15,23 @@ EMU = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpu_emu")
 
 @pytest.fixture(scope="module")
 def built():
-    subprocess.run(["make", "-s", "-C", EMU], check=True)
+    jobs = str(max(1, min(8, os.cpu_count() or 1)))
+    subprocess.run(["make", "-s", "-j", jobs, "-C", EMU], check=True)
     return os.path.join(EMU, "_build")
 
 
-@pytest.mark.parametrize("prog", ["fengine_emu", "xengine_emu", "xmfma_emu"])
-def test_kernel_body_emulation(built, prog):
-    r = subprocess.run([os.path.join(built, prog)], capture_output=True, text=True, timeout=600,
-                       env=dict(os.environ, ASAN_OPTIONS="detect_leaks=0"))
-    print(r.stdout)
-    assert r.returncode == 0, r.stdout + r.stderr
-    assert "OK" in r.stdout
+@pytest.mark.parametrize("prog,parts", [("fengine_emu", 4), ("xengine_emu", 1), ("xmfma_emu", 6)])
+def test_kernel_body_emulation(built, prog, parts):
+    """`parts` processes at once, each running every parts-th check."""
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
+    args = [[]] if parts == 1 else [[str(i), str(parts)] for i in range(parts)]
+    procs = [subprocess.Popen([os.path.join(built, prog)] + a, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                              text=True, env=env) for a in args]
+    for p in procs:
+        out, err = p.communicate(timeout=600)
+        print(out)
+        assert p.returncode == 0, out + err
+        assert "OK" in out
 
 
 def test_generated_code_has_no_clobbered_upper_halves():
