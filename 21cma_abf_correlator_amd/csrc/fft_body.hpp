@@ -125,12 +125,86 @@ CORR_HD CORR_FI void unscramble(float2 *v) {
     }
 }
 
+// In-place 16-point DFT as 4 x 4 (Cooley-Tukey, n = n1 + 4 n2, k = 4 k1 + k2):
+// four radix-4 DFTs over n2, the twiddles W16^(n1 k2), four radix-4 DFTs over
+// n1. Every constant twiddle is applied in a scaled form that an FMA absorbs:
+// W16^1 = C (1 - i T) and W16^3 = C (T - i) (C = cos(pi/8), T = tan(pi/8)) share
+// the factor C, W16^2 = R (1 - i) and W16^6 = R (-1 - i) (R = cos(pi/4)) the
+// factor R, so each column's products are formed unscaled (additions, or one
+// FMA with T per component) and the common factor enters in the column's final
+// FMAs. 144 floating-point instructions where the radix-2 form takes 162.
+CORR_HD CORR_FI void dft16_fma(float2 *v) {
+    constexpr float C = 0.92387953251128675613f;  // cos(pi/8)
+    constexpr float T = 0.41421356237309504880f;  // tan(pi/8)
+    constexpr float R = 0.70710678118654752440f;  // cos(pi/4)
+    float2 y[4][4];  // y[n1][k2]: radix-4 DFT of v[n1 + 4 n2] over n2
+#pragma unroll
+    for (int n1 = 0; n1 < 4; ++n1) {
+        const float2 a0 = v[n1], a1 = v[n1 + 4], a2 = v[n1 + 8], a3 = v[n1 + 12];
+        const float2 t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), t3 = csub(a1, a3);
+        y[n1][0] = cadd(t0, t2);
+        y[n1][2] = csub(t0, t2);
+        y[n1][1] = make_float2(t1.x + t3.y, t1.y - t3.x);  // t1 - i t3
+        y[n1][3] = make_float2(t1.x - t3.y, t1.y + t3.x);  // t1 + i t3
+    }
+    // Column k2: X[4 k1 + k2] = radix-4 DFT over n1 of y[n1][k2] W16^(n1 k2).
+    {  // k2 = 0: no twiddles
+        const float2 t0 = cadd(y[0][0], y[2][0]), t1 = csub(y[0][0], y[2][0]);
+        const float2 t2 = cadd(y[1][0], y[3][0]), t3 = csub(y[1][0], y[3][0]);
+        v[0] = cadd(t0, t2);
+        v[8] = csub(t0, t2);
+        v[4] = make_float2(t1.x + t3.y, t1.y - t3.x);
+        v[12] = make_float2(t1.x - t3.y, t1.y + t3.x);
+    }
+    {  // k2 = 1: y1 W^1 = C (x + T y, y - T x), y2 W^2 = R (x + y, y - x), y3 W^3 = C (y + T x, T y - x)
+        const float2 Y0 = y[0][1], Y1 = y[1][1], Y2 = y[2][1], Y3 = y[3][1];
+        const float qx = Y2.x + Y2.y, qy = Y2.y - Y2.x;
+        const float2 t0 = make_float2(__builtin_fmaf(R, qx, Y0.x), __builtin_fmaf(R, qy, Y0.y));
+        const float2 t1 = make_float2(__builtin_fmaf(-R, qx, Y0.x), __builtin_fmaf(-R, qy, Y0.y));
+        const float2 p1 = make_float2(__builtin_fmaf(T, Y1.y, Y1.x), __builtin_fmaf(-T, Y1.x, Y1.y));
+        const float2 p3 = make_float2(__builtin_fmaf(T, Y3.x, Y3.y), __builtin_fmaf(T, Y3.y, -Y3.x));
+        const float2 u = cadd(p1, p3), w = csub(p1, p3);  // t2 = C u, t3 = C w
+        v[1] = make_float2(__builtin_fmaf(C, u.x, t0.x), __builtin_fmaf(C, u.y, t0.y));
+        v[9] = make_float2(__builtin_fmaf(-C, u.x, t0.x), __builtin_fmaf(-C, u.y, t0.y));
+        v[5] = make_float2(__builtin_fmaf(C, w.y, t1.x), __builtin_fmaf(-C, w.x, t1.y));
+        v[13] = make_float2(__builtin_fmaf(-C, w.y, t1.x), __builtin_fmaf(C, w.x, t1.y));
+    }
+    {  // k2 = 2: y1 W^2 = R (x + y, y - x), y2 W^4 = -i y2, y3 W^6 = R (y - x, -(x + y))
+        const float2 Y0 = y[0][2], Y1 = y[1][2], Y2 = y[2][2], Y3 = y[3][2];
+        const float2 t0 = make_float2(Y0.x + Y2.y, Y0.y - Y2.x), t1 = make_float2(Y0.x - Y2.y, Y0.y + Y2.x);
+        const float q2x = Y1.x + Y1.y, q2y = Y1.y - Y1.x;
+        const float s3 = Y3.x + Y3.y, q6x = Y3.y - Y3.x;
+        const float2 u = make_float2(q2x + q6x, q2y - s3), w = make_float2(q2x - q6x, q2y + s3);  // t2 = R u, t3 = R w
+        v[2] = make_float2(__builtin_fmaf(R, u.x, t0.x), __builtin_fmaf(R, u.y, t0.y));
+        v[10] = make_float2(__builtin_fmaf(-R, u.x, t0.x), __builtin_fmaf(-R, u.y, t0.y));
+        v[6] = make_float2(__builtin_fmaf(R, w.y, t1.x), __builtin_fmaf(-R, w.x, t1.y));
+        v[14] = make_float2(__builtin_fmaf(-R, w.y, t1.x), __builtin_fmaf(R, w.x, t1.y));
+    }
+    {  // k2 = 3: y1 W^3 = C (y + T x, T y - x), y2 W^6 = R (y - x, -(x + y)), y3 W^9 = -C (x + T y, y - T x)
+        const float2 Y0 = y[0][3], Y1 = y[1][3], Y2 = y[2][3], Y3 = y[3][3];
+        const float qx = Y2.y - Y2.x, qy = -(Y2.x + Y2.y);
+        const float2 t0 = make_float2(__builtin_fmaf(R, qx, Y0.x), __builtin_fmaf(R, qy, Y0.y));
+        const float2 t1 = make_float2(__builtin_fmaf(-R, qx, Y0.x), __builtin_fmaf(-R, qy, Y0.y));
+        const float2 p3 = make_float2(__builtin_fmaf(T, Y1.x, Y1.y), __builtin_fmaf(T, Y1.y, -Y1.x));
+        const float2 p1 = make_float2(__builtin_fmaf(T, Y3.y, Y3.x), __builtin_fmaf(-T, Y3.x, Y3.y));
+        const float2 u = csub(p3, p1), w = cadd(p3, p1);  // t2 = C u, t3 = C w
+        v[3] = make_float2(__builtin_fmaf(C, u.x, t0.x), __builtin_fmaf(C, u.y, t0.y));
+        v[11] = make_float2(__builtin_fmaf(-C, u.x, t0.x), __builtin_fmaf(-C, u.y, t0.y));
+        v[7] = make_float2(__builtin_fmaf(C, w.y, t1.x), __builtin_fmaf(-C, w.x, t1.y));
+        v[15] = make_float2(__builtin_fmaf(-C, w.y, t1.x), __builtin_fmaf(C, w.x, t1.y));
+    }
+}
+
 // In-place DFT of R (power of two, <= 32) values, natural order in and out:
 // v[k] := sum_r v[r] exp(-2 pi i r k / R).
 template <int R>
 CORR_HD CORR_FI void dft(float2 *v) {
-    dif_stages<R, R / 2>(v);
-    unscramble<R, 0>(v);
+    if constexpr (R == 16) {
+        dft16_fma(v);
+    } else {
+        dif_stages<R, R / 2>(v);
+        unscramble<R, 0>(v);
+    }
 }
 
 // LDS layouts of the exchange buffer. A pass whose lanes write with a stride
@@ -676,18 +750,16 @@ CORR_HD CORR_FI void pass_twiddle_products(float2 (&w)[16 / R][R - 1]) {
     }
 }
 
-// The N = 16384 transform (passes 16, 16, 16, 4) and the R2C epilogue, all
-// exchanges through the half buffer `lds`. `tab` holds the second pass's
+// The N = 16384 transform after its first pass (v: the second pass's inputs,
+// z'[j + r*N/16], already exchanged): passes 16, 16, 4 and the R2C epilogue,
+// all exchanges through the half buffer `lds`. `tab` holds the second pass's
 // twiddles (fill_tw16, before the first barrier). emit(k, X) stores one bin.
 template <class Sync, class Emit>
-CORR_HD CORR_FI void fengine_half_16384(int j, float2 (&v)[16], float2 *lds, const float2 *tab, Sync &sync,
-                                        const float2 *tw, const float2 *wl, Emit &&emit) {
+CORR_HD CORR_FI void fengine_half_16384_rest(int j, float2 (&v)[16], float2 *lds, const float2 *tab, Sync &sync,
+                                             const float2 *tw, const float2 *wl, Emit &&emit) {
     constexpr int N = 16384;
-    constexpr int NT = N / 16;
-    dft<16>(v);  // first pass: radix 16, NS = 1
     {
         float2 w[1][15];
-        half_exchange<N, 16, 1, 16>(v, lds, j, sync, true);
         read_tw16(tab, j, w[0]);
         pass_compute<N, 16, 16>(v, w);
     }
@@ -698,50 +770,106 @@ CORR_HD CORR_FI void fengine_half_16384(int j, float2 (&v)[16], float2 *lds, con
         pass_twiddle_products<N, 16, 256>(w);
         pass_compute<N, 16, 256>(v, w);
     }
+    // Last pass (radix 4, NS = 4096) and the R2C unpack in registers. Butterfly
+    // b of the last pass makes Z[b + 4096 r], r < 4, and the mirrors N - k of
+    // those bins are the outputs of butterfly 4096 - b (r -> 3 - r). Lane j takes
+    // the butterflies b0 = j, b1 = 4096 - j, b2 = 1024 + j, b3 = 3072 - j (lane 0:
+    // b1 = 2048, the one butterfly no lane pairs), so it holds both bins of each
+    // of its R2C pairs and the spectrum never goes back through LDS.
+    const int b1 = j == 0 ? 2048 : 4096 - j;
+    float2 w0[3];  // W^(j r), r = 1..3 (W = exp(-2 pi i/N)): table w, w^2 and their product
+    w0[0] = tw[(unsigned)j];
+    w0[1] = tw[(unsigned)(2 * j)];
     {
-        float2 w[4][3];
-        pass_twiddle_reads<N, 4, 4096>(tw, j, w);
-        half_exchange<N, 16, 256, 4>(v, lds, j, sync, false);
-        pass_twiddle_products<N, 4, 4096>(w);
-        pass_compute<N, 4, 4096>(v, w);
+        float2 in[16];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            sync();
+            half_write<N, 16, 256>(v, lds, j, h);  // the third pass's outputs, plain layout
+            sync();
+#pragma unroll
+            for (int r = 2 * h; r < 2 * h + 2; ++r) {
+                const int o = r * 4096 - h * (N / 2);
+                in[r] = lds[j + o];
+                in[4 + r] = lds[b1 + o];
+                in[8 + r] = lds[1024 + j + o];
+                in[12 + r] = lds[3072 - j + o];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 16; ++r) v[r] = in[r];
     }
-    // R2C: Z[k] (k < N/2) from half 0, Z[N-k] from half 1 (the last pass wrote
-    // the plain layout).
+    {
+        w0[2] = cmul(w0[0], w0[1]);
+        // b2 = 1024 + j: W^(1024 r) W^(j r), W^(1024 r) = exp(-i pi r/8)
+        float2 w2[3];
+#pragma unroll
+        for (int r = 1; r < 4; ++r) w2[r - 1] = cmul(w0[r - 1], make_float2(cos32(2 * r), -cos32((2 * r + 24) & 31)));
+        // b1 = 4096 - j and b3 = 3072 - j: W^(4096 r) = (-i)^r times the conjugate
+        // of b0's and b2's twiddles; lane 0's b1 = 2048: W^(2048 r) = exp(-i pi r/4)
+        float2 w1[3], w3[3];
+#pragma unroll
+        for (int r = 1; r < 4; ++r) {
+            const float2 c0 = w0[r - 1], c2 = w2[r - 1];
+            // (-i)^r conj(c): r = 1: (-c.y, -c.x); r = 2: (-c.x, c.y); r = 3: (c.y, c.x)
+            w1[r - 1] = r == 1 ? make_float2(-c0.y, -c0.x) : r == 2 ? make_float2(-c0.x, c0.y) : make_float2(c0.y, c0.x);
+            w3[r - 1] = r == 1 ? make_float2(-c2.y, -c2.x) : r == 2 ? make_float2(-c2.x, c2.y) : make_float2(c2.y, c2.x);
+            if (j == 0) w1[r - 1] = make_float2(cos32(4 * r), -cos32((4 * r + 24) & 31));
+        }
+#pragma unroll
+        for (int r = 1; r < 4; ++r) {
+            v[r] = cmul(v[r], w0[r - 1]);
+            v[4 + r] = cmul(v[4 + r], w1[r - 1]);
+            v[8 + r] = cmul(v[8 + r], w2[r - 1]);
+            v[12 + r] = cmul(v[12 + r], w3[r - 1]);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) dft<4>(&v[4 * q]);
+    }
+    // v[4q + r] = Z[b_q + 4096 r]. The pairs (k, N - k): k = j + 4096 r with
+    // N - k = b1 + 4096 (3 - r), and k = 1024 + j + 4096 r with N - k =
+    // b3 + 4096 (3 - r); k = j + 1024 q' for q' = 4r and 4r + 1, so the emitter
+    // and W^k = W^j exp(-i pi q'/16) follow q' (W = exp(-i pi/N)). Lane 0 has
+    // bin 0 (X = Re Z[0] + Im Z[0]), bin N/2 (X = conj Z[N/2]) and the pairs
+    // (4096, 12288), (2048, 14336), (6144, 10240) in place of its first four.
     const float2 wj = wl[(unsigned)j];
-    float2 a[8], b[8], zh = make_float2(0.0f, 0.0f);
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        sync();
-        half_write<N, 4, 4096>(v, lds, j, h);
-        sync();
-        if (h == 0) {
-#pragma unroll
-            for (int q = 0; q < 8; ++q) a[q] = lds[j + q * NT];
-        } else {
-#pragma unroll
-            for (int q = 0; q < 8; ++q) b[q] = lds[(N / 2 - j - q * NT) & (N / 2 - 1)];
-            if (j == 0) zh = lds[0];  // Z[N/2]
-        }
-    }
-    // b[0] of lane 0 is Z[N/2] (read at index 0 of half 1); bin 0 pairs with
-    // itself: X[0] = Re Z[0] + Im Z[0].
     emit.begin();
+    if (j != 0) {
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        const int k = j + q * NT;
-        if (k == 0) {
-            emit.at(0, make_float2(a[0].x + a[0].y, 0.0f));
-        } else {
+        for (int r = 0; r < 4; ++r) {
             float2 xk, xm;
-            r2c_combine(a[q], b[q], half_twiddle(wj, q), xk, xm);
-            emit.pair(q, xk, xm);
+            r2c_combine(v[r], v[4 + 3 - r], half_twiddle(wj, 4 * r), xk, xm);
+            emit.pair(4 * r, xk, xm);
         }
+    } else {
+        float2 xk, xm;
+        emit.at(0, make_float2(v[0].x + v[0].y, 0.0f));
+        emit.at(N / 2, make_float2(v[2].x, -v[2].y));
+        r2c_combine(v[1], v[3], half_twiddle(wj, 4), xk, xm);
+        emit.pair(4, xk, xm);
+        r2c_combine(v[4], v[7], half_twiddle(wj, 2), xk, xm);
+        emit.pair(2, xk, xm);
+        r2c_combine(v[5], v[6], half_twiddle(wj, 6), xk, xm);
+        emit.pair(6, xk, xm);
     }
-    if (j == 0) {
-        // X[N/2] = (Z + conj Z)/2 + W^(N/2) (Z - conj Z)/(2i) with W^(N/2) = -i
-        emit.at(N / 2, make_float2(zh.x, -zh.y));
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        float2 xk, xm;
+        r2c_combine(v[8 + r], v[12 + 3 - r], half_twiddle(wj, 4 * r + 1), xk, xm);
+        emit.pair(4 * r + 1, xk, xm);
     }
 }
+
+// The whole N = 16384 transform with the first pass on the vector ALUs: v holds
+// lane j's inputs z[j + r*N/16].
+template <class Sync, class Emit>
+CORR_HD CORR_FI void fengine_half_16384(int j, float2 (&v)[16], float2 *lds, const float2 *tab, Sync &sync,
+                                        const float2 *tw, const float2 *wl, Emit &&emit) {
+    dft<16>(v);  // first pass: radix 16, NS = 1
+    half_exchange<16384, 16, 1, 16>(v, lds, j, sync, true);
+    fengine_half_16384_rest(j, v, lds, tab, sync, tw, wl, emit);
+}
+
 
 // LDS elements (float2) of the F-engine workgroup at N = 2^LOG2N.
 // The power-of-two F-engine's LDS: the second pass's twiddle table
