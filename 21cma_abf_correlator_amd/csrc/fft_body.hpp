@@ -36,6 +36,18 @@
 namespace corr {
 namespace fft {
 
+#if defined(__HIP_DEVICE_COMPILE__)
+// A raw buffer resource over [base, base + 2 GiB). A load or store through it
+// takes the lane's 32-bit byte offset in a vector register and everything else
+// (base, run offset) in scalar registers, so it needs no vector address
+// arithmetic; a global access with a per-lane 64-bit address costs one or two
+// VALU instructions, which the F-engine (VALU-bound) pays on every access.
+// 0x00020000: the gfx9 descriptor word 3 (32-bit data format, no swizzle).
+CORR_FI __device__ __amdgpu_buffer_rsrc_t raw_rsrc(const void *base) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, 0x7FFFFFFF, 0x00020000);
+}
+#endif
+
 CORR_HD CORR_FI float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 CORR_HD CORR_FI float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 CORR_HD CORR_FI float2 cmul(float2 a, float2 w) {
@@ -413,13 +425,23 @@ CORR_HD CORR_FI float2 int8_pair_to_float2(char2 c) {
 
 // Thread j's 16 input values of frame f of station st, z[j + r*N/16] for
 // r < 16, converted exactly from int8 pairs.
+// Each load is the base of its run r (common to the workgroup) plus the lane's
+// 32-bit byte offset 2j: scalar address arithmetic only.
 template <int LOG2N>
 CORR_HD CORR_FI void load_frame(int j, const int8_t *in, size_t station_pitch, int f, int st, float2 (&v)[16]) {
     constexpr int NT = (1 << LOG2N) / 16;
-    const char2 *src = reinterpret_cast<const char2 *>(in + (size_t)st * station_pitch + (size_t)f * (size_t)(2 << LOG2N));
+    const int8_t *src = in + (size_t)st * station_pitch + (size_t)f * (size_t)(2 << LOG2N);
+    const unsigned oj = 2u * (unsigned)j;
     char2 raw[16];
+#if defined(__HIP_DEVICE_COMPILE__)
+    const __amdgpu_buffer_rsrc_t rs = raw_rsrc(src);
 #pragma unroll
-    for (int r = 0; r < 16; ++r) raw[r] = src[(unsigned)(j + r * NT)];
+    for (int r = 0; r < 16; ++r)
+        raw[r] = __builtin_bit_cast(char2, (unsigned short)__builtin_amdgcn_raw_buffer_load_b16(rs, oj, 2 * r * NT, 0));
+#else
+#pragma unroll
+    for (int r = 0; r < 16; ++r) raw[r] = *reinterpret_cast<const char2 *>(src + 2 * r * NT + oj);
+#endif
 #pragma unroll
     for (int r = 0; r < 16; ++r) v[r] = int8_pair_to_float2(raw[r]);
 }
@@ -568,23 +590,63 @@ struct FormEmit {
     word_t *dst;         // the frame's base in the station's tile-major block
     const float *g;      // the station's gains (not read in the FP32 form)
     unsigned shift, mask, tile_stride;
-    unsigned ok = 0, om = 0;  // off(j), off(N/16 - j), set by begin()
+    // Byte offsets of the lane's bins j and N/16 - j from a base common to the
+    // workgroup: the spectrum words off(j), off(N/16 - j) and the gains j,
+    // N/16 - j. They fit 32 bits (off(N/16) * 8 = N * frame capacity / 2 bytes,
+    // below 2^31 as N * frame capacity < 2^32), so every access in pair() is
+    // a scalar base plus the lane's 32-bit offset: no vector address arithmetic.
+    unsigned ok = 0, om = 0, gk = 0, gm = 0;
     int j;
     CORR_HD CORR_FI unsigned off(unsigned k) const { return (k >> shift) * tile_stride + (k & mask); }
     // The lane offsets are formed here, after the transform, so that they do
     // not hold registers through it.
     CORR_HD CORR_FI void begin() {
-        ok = off((unsigned)j);
-        om = off((unsigned)(N / 16 - j));
+        ok = off((unsigned)j) * (unsigned)sizeof(word_t);
+        om = off((unsigned)(N / 16 - j)) * (unsigned)sizeof(word_t);
+        gk = (unsigned)j * 4u;
+        gm = (unsigned)(N / 16 - j) * 4u;
     }
-    // bin k's word (gain index k): the gain is read only by the requantizing forms
-    CORR_HD CORR_FI word_t word(float2 x, unsigned k) const {
+    // The gain at byte b from base (1 in the FP32 form, which reads none).
+    CORR_HD CORR_FI static float gain(const float *base, unsigned b) {
+        if constexpr (FORM == 0) {
+            return 1.0f;
+        } else {
+#if defined(__HIP_DEVICE_COMPILE__)
+            return __builtin_bit_cast(float, (unsigned)__builtin_amdgcn_raw_buffer_load_b32(raw_rsrc(base), b, 0, 0));
+#else
+            return *reinterpret_cast<const float *>(reinterpret_cast<const char *>(base) + b);
+#endif
+        }
+    }
+    // Store w at byte b from base.
+    CORR_HD CORR_FI static void put(word_t *base, unsigned b, word_t w) {
+#if defined(__HIP_DEVICE_COMPILE__)
+        typedef unsigned u32x2_t __attribute__((ext_vector_type(2)));
+        if constexpr (FORM == 0)
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2_t, w), raw_rsrc(base), b, 0, 0);
+        else if constexpr (FORM == 1)
+            __builtin_amdgcn_raw_buffer_store_b32(w, raw_rsrc(base), b, 0, 0);
+        else
+            __builtin_amdgcn_raw_buffer_store_b16(w, raw_rsrc(base), b, 0, 0);
+#else
+        *reinterpret_cast<word_t *>(reinterpret_cast<char *>(base) + b) = w;
+#endif
+    }
+    // The bin's word with gain gn: the gain is read only by the requantizing forms
+    CORR_HD CORR_FI static word_t word_g(float2 x, float gn) {
         if constexpr (FORM == 0)
             return x;
         else if constexpr (FORM == 1)
-            return forms::pack_fp16(x, g[k]);
+            return forms::pack_fp16(x, gn);
         else
-            return forms::pack_int8(x, g[k]);
+            return forms::pack_int8(x, gn);
+    }
+    // bin k's word (gain index k)
+    CORR_HD CORR_FI word_t word(float2 x, unsigned k) const {
+        if constexpr (FORM == 0)
+            return x;
+        else
+            return word_g(x, g[k]);
     }
     CORR_HD CORR_FI static word_t zero() {
         if constexpr (FORM == 0)
@@ -596,14 +658,14 @@ struct FormEmit {
     CORR_HD CORR_FI void pair(int q, float2 xk, float2 xm) const {
         constexpr int NT = N / 16;
         const unsigned bk = (unsigned)(q * NT), bm = (unsigned)(N - (q + 1) * NT);
-        (dst + off(bk))[ok] = word(xk, bk + (unsigned)j);
-        (dst + off(bm))[om] = word(xm, bm + (unsigned)(NT - j));
+        put(dst + off(bk), ok, word_g(xk, gain(g + bk, gk)));
+        put(dst + off(bm), om, word_g(xm, gain(g + bm, gm)));
     }
     CORR_HD CORR_FI void zero_at(int k) const { dst[off((unsigned)k)] = zero(); }
     CORR_HD CORR_FI void zero_pair(int q) const {
         constexpr int NT = N / 16;
-        (dst + off((unsigned)(q * NT)))[ok] = zero();
-        (dst + off((unsigned)(N - (q + 1) * NT)))[om] = zero();
+        put(dst + off((unsigned)(q * NT)), ok, zero());
+        put(dst + off((unsigned)(N - (q + 1) * NT)), om, zero());
     }
 };
 
