@@ -91,6 +91,17 @@ CORR_HD CORR_FI float2 rot32(float2 x) {
     }
 }
 
+// x * exp(-2 pi i M / 32) for a compile-time M in [0, 32).
+template <int M>
+CORR_HD CORR_FI float2 rot32_any(float2 x) {
+    if constexpr (M >= 16) {
+        const float2 y = rot32<M - 16>(x);
+        return make_float2(-y.x, -y.y);
+    } else {
+        return rot32<M>(x);
+    }
+}
+
 // One radix-2 DIF stage of an in-register DFT of size R: blocks of 2*H.
 template <int R, int H, int BASE, int K>
 CORR_HD CORR_FI void dif_butterflies(float2 *v) {
@@ -547,6 +558,122 @@ CORR_HD CORR_FI void r2c_epilogue(int j, const float2 *lds, const float2 *wl, Em
     if (j == 0) emit.at(N / 2, r2c_bin<N>(lds, wl, N / 2));
 }
 
+// The radix-16 Stockham passes from DONE points combined while 16 or more
+// remain; each ends with a barrier.
+template <int LOG2N, int DONE, bool TAB, class Sync>
+CORR_HD CORR_FI void stockham_radix16_passes(float2 (&v)[16], float2 *lds, const float2 *tw, const float2 *tab, int j,
+                                             Sync &sync) {
+    if constexpr (LOG2N - DONE >= 4) {
+        stockham_pass<(1 << LOG2N), 16, (1 << DONE), TAB>(v, lds, tw, tab, j, sync);
+        stockham_radix16_passes<LOG2N, DONE + 4, TAB>(v, lds, tw, tab, j, sync);
+    }
+}
+
+// The last pass (radix R = 2^(LOG2N mod 4) = 2, 4 or 8, NS = N/R) and the R2C
+// unpack, in registers. Butterfly b makes Z[b + NS r], r < R, and the mirrors
+// N - k of those bins are butterfly NS - b's outputs with r -> R - 1 - r. Lane
+// j takes the NB/2 butterflies b_q = j + (N/16) q and their mirrors NS - b_q
+// (lane 0: NS/2 for b_0 = 0, the one butterfly no lane pairs), so both bins of
+// each of its R2C pairs are in its registers. The mirrors' twiddles are
+// W^(NS r) = exp(-2 pi i r/R) times the conjugates of the direct ones. `lds`
+// holds the previous pass's outputs after its barrier; nothing is written back.
+template <int LOG2N, class Emit>
+CORR_HD CORR_FI void last_pass_r2c(int j, const float2 *lds, const float2 *tw, const float2 *wl, Emit &&emit) {
+    constexpr int N = 1 << LOG2N, NT = N / 16;
+    constexpr int LOGR = LOG2N % 4, R = 1 << LOGR, NB = 16 / R, H = NB / 2, NS = N / R;
+    constexpr bool PIN = pad_for(NS / 16);
+    static_assert(LOGR >= 1, "a last pass of radix 2, 4 or 8");
+    float2 v[16];
+    {
+        const float2 *src = lds + lds_index<PIN>(j);
+        const float2 *srm = lds + lds_index<PIN>(j == 0 ? NS / 2 : NS - j);  // mirror of b_0
+        const float2 *srn = lds + lds_index<PIN>(NS - j);                       // b_q mirrors, q >= 1
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+#pragma unroll
+            for (int q = 0; q < H; ++q) v[q * R + r] = src[lds_span<PIN>(q * NT + r * NS)];
+            v[H * R + r] = srm[lds_span<PIN>(r * NS)];
+#pragma unroll
+            for (int q = 1; q < H; ++q) v[(H + q) * R + r] = srn[lds_span<PIN>(r * NS) - lds_span<PIN>(q * NT)];
+        }
+    }
+    {
+        float2 w0[R - 1];  // W^(j r), W = exp(-2 pi i/N)
+        group_twiddles<R, 1>(tw, j, w0);
+#pragma unroll
+        for (int q = 0; q < H; ++q) {
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                const int m = (2 * r * q) & 31;  // W^(N/16 q r) = exp(-2 pi i m/32)
+                const float2 wd = q == 0 ? w0[r - 1] : cmul(w0[r - 1], make_float2(cos32(m), -cos32((m + 24) & 31)));
+                float2 wm;
+                const float2 c = make_float2(wd.x, -wd.y);
+                switch ((32 / R) * r) {  // W^(NS r) conj(wd), NS r/N = r/R turns
+                    case 2: wm = rot32_any<2>(c); break;
+                    case 4: wm = rot32_any<4>(c); break;
+                    case 6: wm = rot32_any<6>(c); break;
+                    case 8: wm = rot32_any<8>(c); break;
+                    case 10: wm = rot32_any<10>(c); break;
+                    case 12: wm = rot32_any<12>(c); break;
+                    case 14: wm = rot32_any<14>(c); break;
+                    case 16: wm = rot32_any<16>(c); break;
+                    case 20: wm = rot32_any<20>(c); break;
+                    case 24: wm = rot32_any<24>(c); break;
+                    case 28: wm = rot32_any<28>(c); break;
+                    default: wm = c; break;
+                }
+                if (q == 0 && j == 0) {
+                    // butterfly NS/2: W^(NS/2 r) = exp(-i pi r/R)
+                    const int mh = (16 / R) * r;
+                    wm = make_float2(cos32(mh), -cos32((mh + 24) & 31));
+                }
+                v[q * R + r] = cmul(v[q * R + r], wd);
+                v[(H + q) * R + r] = cmul(v[(H + q) * R + r], wm);
+            }
+        }
+#pragma unroll
+        for (int g = 0; g < NB; ++g) dft<R>(&v[g * R]);
+    }
+    // v[q R + r] = Z[b_q + NS r], v[(H + q) R + r] = Z[N - b_q - NS (R - 1 - r)]; the pair of
+    // k = b_q + NS r = j + (N/16)(q + NB r) is emitted as q' = q + NB r.
+    const float2 wj = wl[(unsigned)j];
+    emit.begin();
+    if (j != 0) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            float2 xk, xm;
+            r2c_combine(v[r], v[H * R + R - 1 - r], half_twiddle(wj, NB * r), xk, xm);
+            emit.pair(NB * r, xk, xm);
+        }
+    } else {
+        // bin 0 (X = Re Z[0] + Im Z[0]), bin N/2 (X = conj Z[N/2], Z[N/2] = v[R/2]),
+        // the pairs (NS r, N - NS r) inside butterfly 0 and (NS/2 + NS r, its
+        // mirror) inside butterfly NS/2
+        emit.at(0, make_float2(v[0].x + v[0].y, 0.0f));
+        emit.at(N / 2, make_float2(v[R / 2].x, -v[R / 2].y));
+#pragma unroll
+        for (int r = 1; r < R / 2; ++r) {
+            float2 xk, xm;
+            r2c_combine(v[r], v[R - r], half_twiddle(wj, NB * r), xk, xm);
+            emit.pair(NB * r, xk, xm);
+        }
+#pragma unroll
+        for (int r = 0; r < R / 2; ++r) {
+            float2 xk, xm;
+            r2c_combine(v[H * R + r], v[H * R + R - 1 - r], half_twiddle(wj, H + NB * r), xk, xm);
+            emit.pair(H + NB * r, xk, xm);
+        }
+    }
+#pragma unroll
+    for (int q = 1; q < H; ++q)
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            float2 xk, xm;
+            r2c_combine(v[q * R + r], v[(H + q) * R + R - 1 - r], half_twiddle(wj, q + NB * r), xk, xm);
+            emit.pair(q + NB * r, xk, xm);
+        }
+}
+
 // Whole F-engine workgroup, complex64 output [station][frame][N].
 template <int LOG2N, class Sync, class Emit>
 CORR_HD CORR_FI void fengine_frame(int j, int f, int st, float2 *lds, Sync &sync, const int8_t *in,
@@ -984,8 +1111,24 @@ CORR_HD CORR_FI void fengine_frame(int j, int f, int st, float2 *lds, Sync &sync
         fill_tw16<N>(j, tab, tw);
         fengine_half_16384(j, v, xbuf, tab, sync, tw, wl, emit);
     } else {
-        complex_fft_block<LOG2N>(j, f, st, xbuf, sync, in, station_pitch, tw, tab);
-        r2c_epilogue<N>(j, xbuf, wl, emit);
+        if constexpr (LOG2N == 13) {
+            // first pass, the radix-16 passes, then the last pass and the unpack in
+            // registers (at N = 8192; at N <= 2048, where two to four workgroups share a
+            // CU, it measured slower than the unpack through LDS: more VALU work and
+            // no exchange worth saving)
+            float2 v[16];
+            load_frame<LOG2N>(j, in, station_pitch, f, st, v);
+            fill_tw16<N>(j, tab, tw);
+            dft<16>(v);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) xbuf[17 * j + r] = v[r];  // padded layout: pad16(16 j + r)
+            sync();
+            stockham_radix16_passes<LOG2N, 4, true>(v, xbuf, tw, tab, j, sync);
+            last_pass_r2c<LOG2N>(j, xbuf, tw, wl, emit);
+        } else {
+            complex_fft_block<LOG2N>(j, f, st, xbuf, sync, in, station_pitch, tw, tab);
+            r2c_epilogue<N>(j, xbuf, wl, emit);
+        }
     }
 }
 
