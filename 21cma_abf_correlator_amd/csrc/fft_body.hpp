@@ -646,11 +646,12 @@ CORR_HD CORR_FI void last_pass_r2c(int j, const float2 *lds, const float2 *tw, c
             emit.pair(NB * r, xk, xm);
         }
     } else {
-        // bin 0 (X = Re Z[0] + Im Z[0]), bin N/2 (X = conj Z[N/2], Z[N/2] = v[R/2]),
         // the pairs (NS r, N - NS r) inside butterfly 0 and (NS/2 + NS r, its
-        // mirror) inside butterfly NS/2
-        emit.at(0, make_float2(v[0].x + v[0].y, 0.0f));
-        emit.at(N / 2, make_float2(v[R / 2].x, -v[R / 2].y));
+        // mirror) inside butterfly NS/2, then bin 0 (X = Re Z[0] + Im Z[0]) and
+        // bin N/2 (X = conj Z[N/2], Z[N/2] = v[R/2]). The branches end in stores
+        // of different kinds (buffer stores above, plain stores here): the
+        // compiler then does not merge their last stores after the branch into
+        // one whose base differs between lanes, a loop over the distinct bases.
 #pragma unroll
         for (int r = 1; r < R / 2; ++r) {
             float2 xk, xm;
@@ -663,6 +664,8 @@ CORR_HD CORR_FI void last_pass_r2c(int j, const float2 *lds, const float2 *tw, c
             r2c_combine(v[H * R + r], v[H * R + R - 1 - r], half_twiddle(wj, H + NB * r), xk, xm);
             emit.pair(H + NB * r, xk, xm);
         }
+        emit.at(0, make_float2(v[0].x + v[0].y, 0.0f));
+        emit.at(N / 2, make_float2(v[R / 2].x, -v[R / 2].y));
     }
 #pragma unroll
     for (int q = 1; q < H; ++q)
@@ -733,6 +736,19 @@ struct FormEmit {
         gk = (unsigned)j * 4u;
         gm = (unsigned)(N / 16 - j) * 4u;
     }
+    // An element index that is the same in every lane (a function of the run's
+    // tile parameters and a compile-time bin), read from the first active lane.
+    // That states its uniformity to the compiler where it cannot prove it:
+    // after the branches on the bin-0 lane it may merge the indices of the two
+    // sides, and a buffer access whose base it then takes for a per-lane value
+    // becomes a loop over the wave's distinct bases.
+    CORR_HD CORR_FI static unsigned uniform(unsigned x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+        return __builtin_amdgcn_readfirstlane(x);
+#else
+        return x;
+#endif
+    }
     // The gain at byte b from base (1 in the FP32 form, which reads none).
     CORR_HD CORR_FI static float gain(const float *base, unsigned b) {
         if constexpr (FORM == 0) {
@@ -785,14 +801,14 @@ struct FormEmit {
     CORR_HD CORR_FI void pair(int q, float2 xk, float2 xm) const {
         constexpr int NT = N / 16;
         const unsigned bk = (unsigned)(q * NT), bm = (unsigned)(N - (q + 1) * NT);
-        put(dst + off(bk), ok, word_g(xk, gain(g + bk, gk)));
-        put(dst + off(bm), om, word_g(xm, gain(g + bm, gm)));
+        put(dst + uniform(off(bk)), ok, word_g(xk, gain(g + bk, gk)));
+        put(dst + uniform(off(bm)), om, word_g(xm, gain(g + bm, gm)));
     }
     CORR_HD CORR_FI void zero_at(int k) const { dst[off((unsigned)k)] = zero(); }
     CORR_HD CORR_FI void zero_pair(int q) const {
         constexpr int NT = N / 16;
-        put(dst + off((unsigned)(q * NT)), ok, zero());
-        put(dst + off((unsigned)(N - (q + 1) * NT)), om, zero());
+        put(dst + uniform(off((unsigned)(q * NT))), ok, zero());
+        put(dst + uniform(off((unsigned)(N - (q + 1) * NT))), om, zero());
     }
 };
 
@@ -1031,15 +1047,16 @@ CORR_HD CORR_FI void fengine_half_16384_rest(int j, float2 (&v)[16], float2 *lds
             emit.pair(4 * r, xk, xm);
         }
     } else {
+        // plain stores last, as in last_pass_r2c
         float2 xk, xm;
-        emit.at(0, make_float2(v[0].x + v[0].y, 0.0f));
-        emit.at(N / 2, make_float2(v[2].x, -v[2].y));
         r2c_combine(v[1], v[3], half_twiddle(wj, 4), xk, xm);
         emit.pair(4, xk, xm);
         r2c_combine(v[4], v[7], half_twiddle(wj, 2), xk, xm);
         emit.pair(2, xk, xm);
         r2c_combine(v[5], v[6], half_twiddle(wj, 6), xk, xm);
         emit.pair(6, xk, xm);
+        emit.at(0, make_float2(v[0].x + v[0].y, 0.0f));
+        emit.at(N / 2, make_float2(v[2].x, -v[2].y));
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {

@@ -163,6 +163,25 @@ def lint_kernel(insts):
     return findings
 
 
+def waterfall_loops(insts):
+    """Labels of the blocks that loop over the distinct values of a scalar
+    operand across the wave's lanes (the compiler's form of a memory access
+    whose descriptor or base it could not prove uniform): the block reads a
+    lane's value with v_readfirstlane, masks exec to the lanes that share it
+    (s_and_saveexec), and branches back to itself while lanes remain."""
+    bl, _ = blocks(insts)
+    out = []
+    for lab, ins in bl:
+        if lab is None or not ins:
+            continue
+        last = ins[-1]
+        mns = {mn for mn, _ in ins}
+        if last[0] == "s_cbranch_execnz" and last[1] and last[1][0] == lab and \
+                "v_readfirstlane_b32" in mns and "s_and_saveexec_b64" in mns:
+            out.append(lab)
+    return out
+
+
 def lint_text(text):
     out = []
     for name, insts in parse_kernels(text).items():

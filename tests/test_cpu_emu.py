@@ -48,6 +48,54 @@ def test_generated_code_has_no_clobbered_upper_halves():
     assert not findings, findings
 
 
+def test_fengine_stores_take_scalar_descriptors():
+    """The F-engine's spectrum stores and gain loads are buffer accesses whose
+    base is the same in every lane. Where the compiler cannot prove that (a
+    base merged after the branches on the bin-0 lane), it wraps each access in
+    a loop over the wave's distinct descriptors: about 12 instructions per
+    access on the VALU-bound kernel, and the loop serialises the emission's
+    loads. No F-engine kernel may contain one."""
+    import isa_lint
+    csrc = os.path.join(os.path.dirname(EMU), "..", "21cma_abf_correlator_amd", "csrc")
+    subprocess.run(["make", "-s", "-C", csrc, "asm"], check=True)
+    text = open(os.path.join(csrc, "..", "lib", "obj", "engine_kernels.s")).read()
+    kernels = {k: v for k, v in isa_lint.parse_kernels(text).items()
+               if "fengine_form_kernel" in k or "fengine_fp32_kernel" in k}
+    assert len(kernels) >= 12
+    loops = {k: isa_lint.waterfall_loops(v) for k, v in kernels.items()}
+    assert not any(loops.values()), {k: v for k, v in loops.items() if v}
+
+
+def test_waterfall_detector():
+    """The detector flags the loop form the compiler emits for a buffer store
+    with a per-lane descriptor, and not an ordinary loop."""
+    import isa_lint
+    loop = """k:
+\ts_mov_b64 s[8:9], exec
+.LBB0_1:
+\tv_readfirstlane_b32 s4, v0
+\tv_readfirstlane_b32 s5, v1
+\tv_readfirstlane_b32 s6, v2
+\tv_readfirstlane_b32 s7, v3
+\tv_cmp_eq_u64_e32 vcc, s[4:5], v[0:1]
+\tv_cmp_eq_u64_e64 s[0:1], s[6:7], v[2:3]
+\ts_and_b64 s[0:1], vcc, s[0:1]
+\ts_and_saveexec_b64 s[0:1], s[0:1]
+\tbuffer_store_dword v16, v26, s[4:7], 0 offen
+\ts_xor_b64 exec, exec, s[0:1]
+\ts_cbranch_execnz .LBB0_1
+\ts_mov_b64 exec, s[8:9]
+\ts_endpgm
+.Lfunc_end0:
+"""
+    (insts,) = isa_lint.parse_kernels(loop).values()
+    assert isa_lint.waterfall_loops(insts) == [".LBB0_1"]
+    plain = loop.replace("\tv_readfirstlane_b32 s4, v0\n", "").replace("\tv_readfirstlane_b32 s5, v1\n", "") \
+        .replace("\tv_readfirstlane_b32 s6, v2\n", "").replace("\tv_readfirstlane_b32 s7, v3\n", "")
+    (insts,) = isa_lint.parse_kernels(plain).values()
+    assert isa_lint.waterfall_loops(insts) == []
+
+
 def test_isa_lint_catches_the_known_miscompile():
     """A load result reused as the upper half of a zero-extended index, on one
     path only (the pattern ROCm 7.2's allocator produced for the F-engine)."""
