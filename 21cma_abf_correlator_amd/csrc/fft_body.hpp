@@ -863,7 +863,14 @@ CORR_HD CORR_FI void half_write(const float2 (&v)[16], float2 *lds, int j, int h
         const int k = j & (NS - 1);
         const int base = (j - k) * 16 + k;  // < N/2 exactly when the lane's writes are
         if ((base >= N / 2) == (h == 1)) {
-            float2 *dst = lds + lds_index<POUT>(base - h * (N / 2));
+            int at = lds_index<POUT>(base - h * (N / 2));
+#if defined(__HIP_DEVICE_COMPILE__)
+            // a base of its own for each half: derived from the other half's,
+            // the second half's stores would take offsets below it (negative,
+            // which a ds_write's immediate cannot hold) and one address add each
+            asm volatile("" : "+v"(at));
+#endif
+            float2 *dst = lds + at;
 #pragma unroll
             for (int r = 0; r < 16; ++r) dst[NS == 1 ? r : lds_span<POUT>(r * NS)] = v[r];
         }
