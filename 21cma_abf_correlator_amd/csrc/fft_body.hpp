@@ -638,6 +638,16 @@ CORR_HD CORR_FI void last_pass_r2c(int j, const float2 *lds, const float2 *tw, c
     // k = b_q + NS r = j + (N/16)(q + NB r) is emitted as q' = q + NB r.
     const float2 wj = wl[(unsigned)j];
     emit.begin();
+    // the butterflies every lane has alike first, so that their stores' offsets
+    // are formed before the branch on the bin-0 lane, not on both of its sides
+#pragma unroll
+    for (int q = 1; q < H; ++q)
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            float2 xk, xm;
+            r2c_combine(v[q * R + r], v[(H + q) * R + R - 1 - r], half_twiddle(wj, q + NB * r), xk, xm);
+            emit.pair(q + NB * r, xk, xm);
+        }
     if (j != 0) {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -648,33 +658,26 @@ CORR_HD CORR_FI void last_pass_r2c(int j, const float2 *lds, const float2 *tw, c
     } else {
         // the pairs (NS r, N - NS r) inside butterfly 0 and (NS/2 + NS r, its
         // mirror) inside butterfly NS/2, then bin 0 (X = Re Z[0] + Im Z[0]) and
-        // bin N/2 (X = conj Z[N/2], Z[N/2] = v[R/2]). The branches end in stores
-        // of different kinds (buffer stores above, plain stores here): the
-        // compiler then does not merge their last stores after the branch into
-        // one whose base differs between lanes, a loop over the distinct bases.
+        // bin N/2 (X = conj Z[N/2], Z[N/2] = v[R/2]). Lane 0 stores them with
+        // plain stores only: the branches then share no store the compiler
+        // could merge after them into one whose base differs between lanes.
 #pragma unroll
         for (int r = 1; r < R / 2; ++r) {
             float2 xk, xm;
             r2c_combine(v[r], v[R - r], half_twiddle(wj, NB * r), xk, xm);
-            emit.pair(NB * r, xk, xm);
+            emit.at(NB * r * NT, xk);
+            emit.at(N - NB * r * NT, xm);
         }
 #pragma unroll
         for (int r = 0; r < R / 2; ++r) {
             float2 xk, xm;
             r2c_combine(v[H * R + r], v[H * R + R - 1 - r], half_twiddle(wj, H + NB * r), xk, xm);
-            emit.pair(H + NB * r, xk, xm);
+            emit.at((H + NB * r) * NT, xk);
+            emit.at(N - (H + NB * r) * NT, xm);
         }
         emit.at(0, make_float2(v[0].x + v[0].y, 0.0f));
         emit.at(N / 2, make_float2(v[R / 2].x, -v[R / 2].y));
     }
-#pragma unroll
-    for (int q = 1; q < H; ++q)
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            float2 xk, xm;
-            r2c_combine(v[q * R + r], v[(H + q) * R + R - 1 - r], half_twiddle(wj, q + NB * r), xk, xm);
-            emit.pair(q + NB * r, xk, xm);
-        }
 }
 
 // Whole F-engine workgroup, complex64 output [station][frame][N].
@@ -1054,14 +1057,17 @@ CORR_HD CORR_FI void fengine_half_16384_rest(int j, float2 (&v)[16], float2 *lds
             emit.pair(4 * r, xk, xm);
         }
     } else {
-        // plain stores last, as in last_pass_r2c
+        // plain stores only, as in last_pass_r2c
         float2 xk, xm;
         r2c_combine(v[1], v[3], half_twiddle(wj, 4), xk, xm);
-        emit.pair(4, xk, xm);
+        emit.at(4096, xk);
+        emit.at(N - 4096, xm);
         r2c_combine(v[4], v[7], half_twiddle(wj, 2), xk, xm);
-        emit.pair(2, xk, xm);
+        emit.at(2048, xk);
+        emit.at(N - 2048, xm);
         r2c_combine(v[5], v[6], half_twiddle(wj, 6), xk, xm);
-        emit.pair(6, xk, xm);
+        emit.at(6144, xk);
+        emit.at(N - 6144, xm);
         emit.at(0, make_float2(v[0].x + v[0].y, 0.0f));
         emit.at(N / 2, make_float2(v[2].x, -v[2].y));
     }
