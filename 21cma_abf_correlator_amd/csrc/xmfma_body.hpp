@@ -229,14 +229,28 @@ struct Staging<0, CH, KB> {  // FP32: 4 frames x 32 B (4 complex64 channels)
 
     // Row 2s of channel m holds the real parts of frames 4o..4o+3, row 2s+1 the
     // imaginary parts: w[8i + 2m] and w[8i + 2m + 1] of frame i, no arithmetic.
+    // Each word is copied into its row here, where it is consumed: with the
+    // words in the rows' order from the start, the compiler copies them there
+    // right after their loads, which then wait for their data at once instead
+    // of staying in flight through the MFMA phase.
+    static CORR_HD inline uint32_t at_store(uint32_t x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+        uint32_t y;
+        asm volatile("v_mov_b32 %0, %1" : "=v"(y) : "v"(x));
+        return y;
+#else
+        return x;
+#endif
+    }
     static CORR_HD inline void store(int item, uint8_t *lds, size_t cpitch, const Stage &st) {
         int q, o, s;
         item_coords<CH, KB>(item, q, o, s);
 #pragma unroll
         for (int m = 0; m < 4; ++m) {
-            const uint4 re = make_uint4(st.w[2 * m], st.w[8 + 2 * m], st.w[16 + 2 * m], st.w[24 + 2 * m]);
-            const uint4 im =
-                make_uint4(st.w[2 * m + 1], st.w[8 + 2 * m + 1], st.w[16 + 2 * m + 1], st.w[24 + 2 * m + 1]);
+            const uint4 re = make_uint4(at_store(st.w[2 * m]), at_store(st.w[8 + 2 * m]), at_store(st.w[16 + 2 * m]),
+                                        at_store(st.w[24 + 2 * m]));
+            const uint4 im = make_uint4(at_store(st.w[2 * m + 1]), at_store(st.w[8 + 2 * m + 1]),
+                                        at_store(st.w[16 + 2 * m + 1]), at_store(st.w[24 + 2 * m + 1]));
             uint8_t *row = lds + (size_t)(4 * q + m) * cpitch + (size_t)(2 * s) * kRowBytes +
                            16 * (o ^ slot_swizzle(4 * q + m));
             *reinterpret_cast<uint4 *>(row) = re;
