@@ -210,7 +210,7 @@ struct XShape {
 };
 std::mutex g_tile_mu;
 std::map<TileKey, XShape> g_tile_cache;
-constexpr int kTuneRounds = 5;
+constexpr int kTuneRounds = 7;
 constexpr int kTuneFrames = 2048;
 }  // namespace
 
@@ -231,10 +231,12 @@ int Engine::frames_per_block() const {
 // barriers per frame and put more bytes in flight per step, at the price of
 // LDS and registers). Every combination that fits the shape is timed on the
 // engine's own buffers: F-engine over min(chunk_frames, kTuneFrames) frames of
-// zeros and X-engine over them, the launch size itself (at fewer frames the
+// hashed int8 noise (zeros draw less power, so they run at a higher clock and
+// favoured the wrong candidate) and X-engine over them, the launch size itself (at fewer frames the
 // X-engine's fixed cost per launch hides what the spectrum order changes); one
-// warm-up round, then the fastest of kTuneRounds rounds timed one by one, so
-// that one disturbed round does not decide. The fastest candidate wins. The
+// warm-up round of each, then kTuneRounds rounds in which every candidate takes
+// its turn, timed one by one; a candidate's time is its fastest round, so that
+// one disturbed round does not decide. The fastest candidate wins. The
 // choice is cached per (form, inputs, n_freq, frames timed, device).
 corr_status Engine::tune_xtile() {
     const int N = shape_.n_station, nf = shape_.n_freq;
@@ -301,7 +303,7 @@ corr_status Engine::tune_xtile() {
         (st = hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate")) == CORR_OK &&
         (st = hip_check(hipEventCreate(&e0), "hipEventCreate")) == CORR_OK &&
         (st = hip_check(hipEventCreate(&e1), "hipEventCreate")) == CORR_OK &&
-        (st = hip_check(launch_fill(in, in_bytes / 4, 0u, s), "fill_kernel(input)")) == CORR_OK &&
+        (st = hip_check(launch_noise(in, in_bytes / 4, s), "noise_kernel(input)")) == CORR_OK &&
         (st = hip_check(launch_fill(integ, integ_bytes / 4, 0u, s), "fill_kernel(integrator)")) == CORR_OK) {
         auto round = [&](const XShape &c) {
             corr_status r = transform((int)shape_.form, static_cast<const int8_t *>(in), 0, nullptr, 0, frames,
@@ -311,24 +313,27 @@ corr_status Engine::tune_xtile() {
                                                  nf, c.tile, frames, integ, s),
                              "xengine_mfma_kernel");
         };
-        for (const auto &c : cand) {
-            if ((st = round(c)) != CORR_OK) break;  // warm-up
-            float cand_ms = 0.0f;
-            for (int r = 0; r < kTuneRounds && st == CORR_OK; ++r) {
+        // the candidates take turns round by round, so that a drift of the clock
+        // over the tuning (the part heats up) reaches all of them alike
+        std::vector<float> cand_ms(cand.size(), 0.0f);
+        for (size_t i = 0; i < cand.size() && st == CORR_OK; ++i) st = round(cand[i]);  // warm-up
+        for (int r = 0; r < kTuneRounds && st == CORR_OK; ++r)
+            for (size_t i = 0; i < cand.size() && st == CORR_OK; ++i) {
                 float ms = 0.0f;
-                if ((st = hip_check(hipEventRecord(e0, s), "hipEventRecord")) != CORR_OK || (st = round(c)) != CORR_OK ||
+                if ((st = hip_check(hipEventRecord(e0, s), "hipEventRecord")) != CORR_OK ||
+                    (st = round(cand[i])) != CORR_OK ||
                     (st = hip_check(hipEventRecord(e1, s), "hipEventRecord")) != CORR_OK ||
                     (st = hip_check(hipEventSynchronize(e1), "hipEventSynchronize")) != CORR_OK ||
                     (st = hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime")) != CORR_OK)
                     break;
-                if (r == 0 || ms < cand_ms) cand_ms = ms;
+                if (r == 0 || ms < cand_ms[i]) cand_ms[i] = ms;
             }
-            if (st != CORR_OK) break;
-            if (c == cand[0] || cand_ms < best_ms) {
-                best_ms = cand_ms;
-                best = c;
-            }
-        }
+        if (st == CORR_OK)
+            for (size_t i = 0; i < cand.size(); ++i)
+                if (i == 0 || cand_ms[i] < best_ms) {
+                    best_ms = cand_ms[i];
+                    best = cand[i];
+                }
     }
     if (s) hipStreamSynchronize(s);
     if (e0) hipEventDestroy(e0);

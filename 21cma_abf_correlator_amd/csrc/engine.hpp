@@ -252,6 +252,8 @@ hipError_t launch_power(int nf, int frames, int n_station, const float2 *spec,
 hipError_t launch_gains(int form, int nf, int n_station, const float *power, const int32_t *count,
                         float *gain, hipStream_t s);
 hipError_t launch_fill(void *p, size_t n_words, uint32_t v, hipStream_t s);
+/// n_words of the autotuner's synthetic int8 input (fft::tuning_noise_word).
+hipError_t launch_noise(void *p, size_t n_words, hipStream_t s);
 /// The matrix-core X-engine of `form` with ch channels per workgroup and kb
 /// MFMA K blocks per staged step (xmfma::tile_fits): prepare raises its LDS
 /// limit, launch adds `frames` frames of the spectra into the integrator.

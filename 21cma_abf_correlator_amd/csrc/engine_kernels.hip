@@ -353,6 +353,11 @@ __global__ void __launch_bounds__(kChanBlock) fill_kernel(uint32_t *p, size_t n,
     for (size_t i = (size_t)blockIdx.x * kChanBlock + threadIdx.x; i < n; i += total) p[i] = v;
 }
 
+// The autotuner's synthetic input: n words of fft::tuning_noise_word (grid-stride).
+__global__ void __launch_bounds__(kChanBlock) noise_kernel(uint32_t *p, size_t n, size_t total) {
+    for (size_t i = (size_t)blockIdx.x * kChanBlock + threadIdx.x; i < n; i += total) p[i] = fft::tuning_noise_word(i);
+}
+
 // Dump of one integration into complex64 [baseline][nf]; grid (ceil(nf/256), baselines).
 //   FORM 0: acc * (1.0f/frames) in FP32 (the reference's row scale)
 //   FORM 1: ((double)sum + (double)compensation) / (g_j g_k frames) in FP64
@@ -646,6 +651,15 @@ hipError_t launch_fill(void *p, size_t n_words, uint32_t v, hipStream_t s) {
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(fill_kernel, dim3((unsigned)blocks), dim3(kChanBlock), 0, s,
                        static_cast<uint32_t *>(p), n_words, v, blocks * kChanBlock);
+    return hipGetLastError();
+}
+
+hipError_t launch_noise(void *p, size_t n_words, hipStream_t s) {
+    if (n_words == 0) return hipSuccess;
+    size_t blocks = (n_words + kChanBlock - 1) / kChanBlock;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(noise_kernel, dim3((unsigned)blocks), dim3(kChanBlock), 0, s, static_cast<uint32_t *>(p), n_words,
+                       blocks * kChanBlock);
     return hipGetLastError();
 }
 

@@ -48,6 +48,22 @@ CORR_FI __device__ __amdgpu_buffer_rsrc_t raw_rsrc(const void *base) {
 }
 #endif
 
+// Word i of the autotuner's synthetic input: four int8 samples in [-64, 63]
+// from a hash of i (an integer mix in the manner of MurmurHash3's finalizer).
+// A transform of noise draws the power, and so runs at the clock, that real
+// samples do; one of zeros draws less and would time the candidates faster.
+CORR_HD CORR_FI uint32_t tuning_noise_word(uint64_t i) {
+    uint32_t h = (uint32_t)i ^ (uint32_t)(i >> 32) * 0x85EBCA6Bu;
+    h ^= h >> 16;
+    h *= 0x7FEB352Du;
+    h ^= h >> 15;
+    h *= 0x846CA68Bu;
+    h ^= h >> 16;
+    uint32_t w = 0;
+    for (int k = 0; k < 4; ++k) w |= (uint32_t)(uint8_t)((int8_t)(uint8_t)(h >> (8 * k)) >> 1) << (8 * k);
+    return w;
+}
+
 CORR_HD CORR_FI float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 CORR_HD CORR_FI float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 CORR_HD CORR_FI float2 cmul(float2 a, float2 w) {

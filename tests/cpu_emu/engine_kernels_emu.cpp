@@ -349,6 +349,12 @@ hipError_t launch_fill(void *p, size_t n_words, uint32_t v, hipStream_t) {
     return hipSuccess;
 }
 
+hipError_t launch_noise(void *p, size_t n_words, hipStream_t) {
+    uint32_t *w = static_cast<uint32_t *>(p);
+    for (size_t i = 0; i < n_words; ++i) w[i] = fft::tuning_noise_word(i);
+    return hipSuccess;
+}
+
 hipError_t xengine_mfma_prepare(int form, int n_station, int ch, int kb) {
     return (form >= 0 && form <= 2) && xmfma::tile_fits(n_station, ch, kb) ? hipSuccess : hipErrorInvalidValue;
 }
