@@ -231,8 +231,9 @@ int Engine::frames_per_block() const {
 // barriers per frame and put more bytes in flight per step, at the price of
 // LDS and registers). Every combination that fits the shape is timed on the
 // engine's own buffers: F-engine over min(chunk_frames, kTuneFrames) frames of
-// hashed int8 noise (zeros draw less power, so they run at a higher clock and
-// favoured the wrong candidate) and X-engine over them, the launch size itself (at fewer frames the
+// hashed int8 noise (a transform of zeros draws less power than one of real
+// samples, and the part's clock follows its power) and X-engine over them, the
+// launch size itself (at fewer frames the
 // X-engine's fixed cost per launch hides what the spectrum order changes); one
 // warm-up round of each, then kTuneRounds rounds in which every candidate takes
 // its turn, timed one by one; a candidate's time is its fastest round, so that

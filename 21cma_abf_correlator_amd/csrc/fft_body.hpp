@@ -50,8 +50,8 @@ CORR_FI __device__ __amdgpu_buffer_rsrc_t raw_rsrc(const void *base) {
 
 // Word i of the autotuner's synthetic input: four int8 samples in [-64, 63]
 // from a hash of i (an integer mix in the manner of MurmurHash3's finalizer).
-// A transform of noise draws the power, and so runs at the clock, that real
-// samples do; one of zeros draws less and would time the candidates faster.
+// A transform of noise draws about the power that real samples do, and the
+// part's clock follows its power; one of zeros draws less.
 CORR_HD CORR_FI uint32_t tuning_noise_word(uint64_t i) {
     uint32_t h = (uint32_t)i ^ (uint32_t)(i >> 32) * 0x85EBCA6Bu;
     h ^= h >> 16;
