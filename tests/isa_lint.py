@@ -12,7 +12,12 @@ buffers 2^32 * mask_byte elements out of bounds.
 The check follows every control-flow path from each load (basic blocks split at
 labels and branches) until the loaded register is redefined.
 
-usage: python tests/isa_lint.py file.s [file.s ...]   (exit 1 on a finding)
+`waterfall_loops` finds a second, performance-only pattern: a block that loops
+over the distinct values of a scalar operand across the wave's lanes, the form
+the compiler gives a buffer access whose descriptor it cannot prove uniform
+(tests/test_cpu_emu.py requires none in the F-engine kernels).
+
+usage: python tests/isa_lint.py file.s [file.s ...]   (exit 1 on a hazard finding)
 """
 
 import re
