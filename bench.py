@@ -376,6 +376,21 @@ def traced_kernels(N, nf, chunk):
     return t.get("avg_us", {}), os.path.relpath(path, ROOT)
 
 
+def valu_issue(N, nf, chunk):
+    """VALU issue figures of the F-engines (instructions per CU-cycle and per
+    wave and frame, clock) from the committed rocprofv3 counter summary of this
+    workload; empty when there is none for this geometry."""
+    path = os.path.join(ROOT, "profiles", f"fx_{N}in_nf{nf}_util.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except OSError:
+        return {}, None
+    if t.get("workload") != f"{N} inputs, n_freq {nf}, {chunk} frames per launch":
+        return {}, None
+    return t, os.path.relpath(path, ROOT)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -577,6 +592,11 @@ def main():
         roof["profile"] = {"avg_us": us, "achieved": round(ach, 2 if roof["unit"] == "TFLOP/s" else 1),
                            "frac": round(ach / roof["peak"], 4), "source": source,
                            "event_over_trace": round(kk["avg_ms"] * 1e3 / us, 4)}
+    # an F-engine is bound by VALU issue, not by HBM: about one VALU instruction
+    # per CU and cycle is what a CU issues (a probe of independent FMA chains)
+    valu, vsource = valu_issue(N, nf, h["chunk"])
+    if roof["kernel"] in valu:
+        roof["valu_issue"] = dict(valu[roof["kernel"]], peak_per_cu_cycle=1.0, source=vsource)
     value = h["value"]
     ms_per_step = h["ms_per_step"]
     kernels = h["kernels"]

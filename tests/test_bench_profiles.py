@@ -34,6 +34,21 @@ def test_profiles_match_the_default_workload(form):
     assert bench.measured_traffic(24, 16384, 512, form) == {}
 
 
+def test_valu_issue_profile():
+    """The committed F-engine VALU figures are of the default workload and
+    plausible: at most about one VALU instruction per CU-cycle (a CU's issue
+    rate), about a thousand per wave and frame."""
+    import bench
+    v, src = bench.valu_issue(24, 16384, 2048)
+    assert src == os.path.join("profiles", "fx_24in_nf16384_util.json")
+    for form in ("fp16", "fp32", "int8"):
+        f = v[f"fengine_{form}"]
+        assert 0.5 < f["valu_per_cu_cycle"] <= 1.15
+        assert 800 < f["valu_per_wave_and_frame"] < 1300
+        assert 1.0 < f["clock_ghz"] < 2.5
+    assert bench.valu_issue(40, 16384, 2048) == ({}, None)
+
+
 def test_traffic_near_the_algorithmic_bytes():
     """The calibrated PMC bytes of each kernel are its algorithmic bytes to
     within 2% (no operand re-read from HBM)."""
