@@ -1266,54 +1266,40 @@ CORR_HD CORR_FI void dft_odd(float2 (&a)[M]) {
 // z[n1 + P n2] of the frame (int8 pairs, exact in float), and writes
 // y_k2[n1] = DFT_M(...)[k2] * W_N^(n1 k2) at offset n1 of span k2 (plain
 // layout). twn[m] = exp(-2 pi i m/N), m < N; n1 k2 < N.
-// At n_freq 6144 every load of the lane's NI columns is issued before the
-// first is used, so that their latencies overlap (15-22% faster there); at
-// 10240, 12288 and 14336 that measured 1-11% slower than a column at a time.
+// Every load of the lane's NI columns is issued before the first is used, so
+// that their latencies overlap (a column at a time, each waits for its own),
+// and a column reads one twiddle, W_N^n1, whose powers W_N^(n1 k2), k2 < M,
+// are formed by products (M - 2 complex products, each adding a rounding of
+// about 2^-24): M - 1 scattered table reads per column took longer.
 template <int M, int LOG2P>
 CORR_HD CORR_FI void mixed_first_stage(int j, const char2 *src, float2 *lds, const float2 *twn) {
     using S = MixedSizes<M, LOG2P>;
     constexpr int NI = (S::P + S::kThreads - 1) / S::kThreads;
-    if constexpr (M == 3 && LOG2P == 11) {
-        char2 c[NI][M];
-        float2 w[NI][M - 1];
+    char2 c[NI][M];
+    float2 w[NI];
 #pragma unroll
-        for (int i = 0; i < NI; ++i) {
-            const int n1 = j + i * S::kThreads;
-            if (i + 1 < NI || n1 < S::P) {  // only the last column can lie past P
+    for (int i = 0; i < NI; ++i) {
+        const int n1 = j + i * S::kThreads;
+        if (i + 1 < NI || n1 < S::P) {  // only the last column can lie past P
 #pragma unroll
-                for (int n2 = 0; n2 < M; ++n2) c[i][n2] = src[(unsigned)(n1 + n2 * S::P)];
-#pragma unroll
-                for (int k2 = 1; k2 < M; ++k2) w[i][k2 - 1] = twn[(unsigned)(n1 * k2)];
-            }
+            for (int n2 = 0; n2 < M; ++n2) c[i][n2] = src[(unsigned)(n1 + n2 * S::P)];
+            w[i] = twn[(unsigned)n1];
         }
+    }
 #pragma unroll
-        for (int i = 0; i < NI; ++i) {
-            const int n1 = j + i * S::kThreads;
-            if (i + 1 < NI || n1 < S::P) {
-                float2 a[M];
+    for (int i = 0; i < NI; ++i) {
+        const int n1 = j + i * S::kThreads;
+        if (i + 1 < NI || n1 < S::P) {
+            float2 a[M];
 #pragma unroll
-                for (int n2 = 0; n2 < M; ++n2) a[n2] = make_float2((float)c[i][n2].x, (float)c[i][n2].y);
-                dft_odd<M>(a);
-                lds[n1] = a[0];
+            for (int n2 = 0; n2 < M; ++n2) a[n2] = make_float2((float)c[i][n2].x, (float)c[i][n2].y);
+            dft_odd<M>(a);
+            lds[n1] = a[0];
+            float2 wk = w[i];  // W_N^(n1 k2)
 #pragma unroll
-                for (int k2 = 1; k2 < M; ++k2) lds[k2 * S::kSub + n1] = cmul(a[k2], w[i][k2 - 1]);
-            }
-        }
-    } else {
-#pragma unroll
-        for (int i = 0; i < NI; ++i) {
-            const int n1 = j + i * S::kThreads;
-            if (n1 < S::P) {
-                float2 a[M];
-#pragma unroll
-                for (int n2 = 0; n2 < M; ++n2) {
-                    const char2 c = src[(unsigned)(n1 + n2 * S::P)];
-                    a[n2] = make_float2((float)c.x, (float)c.y);
-                }
-                dft_odd<M>(a);
-                lds[n1] = a[0];
-#pragma unroll
-                for (int k2 = 1; k2 < M; ++k2) lds[k2 * S::kSub + n1] = cmul(a[k2], twn[(unsigned)(n1 * k2)]);
+            for (int k2 = 1; k2 < M; ++k2) {
+                lds[k2 * S::kSub + n1] = cmul(a[k2], wk);
+                if (k2 + 1 < M) wk = cmul(wk, w[i]);
             }
         }
     }
