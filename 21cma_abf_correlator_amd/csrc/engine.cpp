@@ -58,9 +58,9 @@ int fft_twiddle_elems(int n_freq) {
     return p == n_freq ? n_freq : n_freq + p;
 }
 
-// Frames per batch of the staged transform: at most 64, and fewer where its two
-// buffers (8 bytes per retained channel, input and frame, each) would pass
-// 1 GiB each (at 24 inputs: 64 frames up to n_freq 65536, 5 at 2^20).
+// Frames per batch of the staged transform: at most 64, and fewer where its
+// buffer (8 bytes per retained channel, input and frame) would pass 1 GiB (at
+// 24 inputs: 64 frames up to n_freq 65536, 5 at 2^20).
 constexpr int kStageFrames = 64;
 constexpr size_t kStageBytes = size_t(1) << 30;
 
@@ -122,10 +122,8 @@ corr_status Engine::init() {
         stage_frames_ = shape_.chunk_frames < kStageFrames ? shape_.chunk_frames : kStageFrames;
         if ((size_t)stage_frames_ > fit) stage_frames_ = (int)fit;
         const size_t stage = sizeof(float2) * (size_t)N * (size_t)stage_frames_ * (size_t)nf;
-        if ((st = hip_check(hipMalloc(reinterpret_cast<void **>(&d_stage_y_), stage), "hipMalloc(staged transform)")) !=
-                CORR_OK ||
-            (st = hip_check(hipMalloc(reinterpret_cast<void **>(&d_stage_z_), stage), "hipMalloc(staged transform)")) !=
-                CORR_OK)
+        if ((st = hip_check(hipMalloc(reinterpret_cast<void **>(&d_stage_), stage), "hipMalloc(staged transform)")) !=
+            CORR_OK)
             return st;
     }
     if ((st = hip_check(hipMalloc(reinterpret_cast<void **>(&d_valid_),
@@ -400,8 +398,7 @@ Engine::~Engine() {
         if (ev_x_[i] != nullptr) hipEventDestroy(ev_x_[i]);
     }
     hipFree(d_spec_b_);
-    hipFree(d_stage_y_);
-    hipFree(d_stage_z_);
+    hipFree(d_stage_);
     hipFree(d_tw_);
     hipFree(d_wl_);
     hipFree(d_spec_);
@@ -542,12 +539,12 @@ corr_status Engine::transform(int form, const int8_t *d_in, size_t station_pitch
                               size_t valid_stride, int frames, void *out, size_t out_station_stride, int tile,
                               hipStream_t s) {
     const int N = shape_.n_station, nf = shape_.n_freq;
-    if (d_stage_y_ != nullptr) {
+    if (d_stage_ != nullptr) {
         for (int f0 = 0; f0 < frames; f0 += stage_frames_) {
             const int n = frames - f0 < stage_frames_ ? frames - f0 : stage_frames_;
             corr_status st = hip_check(launch_fengine_staged(form, nf, d_in, station_pitch, valid, valid_stride, f0, n,
                                                              N, d_tw_, d_wl_, d_gain_, out, out_station_stride, tile,
-                                                             d_stage_y_, d_stage_z_, s),
+                                                             d_stage_, s),
                                        "staged transform kernels");
             if (st != CORR_OK) return st;
         }

@@ -197,9 +197,9 @@ class Engine {
     float2 *d_wl_ = nullptr;    ///< exp(-2 pi i k / (2 n_freq)), k < n_freq
     void *d_spec_ = nullptr;
     int32_t *d_valid_ = nullptr;  ///< expanded validity, [station][chunk_frames]
-    /// n_freq above 16384: the staged transform's buffers, [station][frame][n_freq]
-    /// complex64 for stage_frames_ frames each
-    float2 *d_stage_y_ = nullptr, *d_stage_z_ = nullptr;
+    /// n_freq above 16384: the staged transform's buffer, [station][frame][n_freq]
+    /// complex64 for stage_frames_ frames
+    float2 *d_stage_ = nullptr;
     int stage_frames_ = 0;
     int2 *d_pairs_ = nullptr;     ///< (j, k) of every baseline
     float *d_gain_ = nullptr;     ///< FP16/int8 forms: [station][n_freq]
@@ -218,13 +218,13 @@ class Engine {
 // Kernel launchers (engine_kernels.hip).
 /// 4-bit packed frames (n_freq bytes each, in_pitch bytes per station) to int8
 /// frames (2*n_freq bytes, out_pitch per station), `frames` frames of each station.
-/// n_freq above 16384: stages 1-3 of the staged transform for `frames` frames
+/// n_freq above 16384: both stages of the staged transform for `frames` frames
 /// starting at frame0 (input, validity and output are indexed by frame0 + f;
-/// ys, zs hold [station][frames][n_freq] complex64).
+/// zs holds [station][frames][n_freq] complex64).
 hipError_t launch_fengine_staged(int form, int n_freq, const int8_t *in, size_t station_pitch, const int32_t *valid,
                                  size_t valid_stride, int frame0, int frames, int n_station, const float2 *tw,
                                  const float2 *wl, const float *gain, void *out, size_t out_station_stride, int tile,
-                                 float2 *ys, float2 *zs, hipStream_t s);
+                                 float2 *zs, hipStream_t s);
 hipError_t launch_unpack4(const int8_t *in, size_t in_pitch, int8_t *out, size_t out_pitch, int frames,
                           int n_freq, int n_station, hipStream_t s);
 /// tw: fft_twiddle_elems(n_freq) entries; wl: n_freq.

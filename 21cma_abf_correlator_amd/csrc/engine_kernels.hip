@@ -144,36 +144,28 @@ fengine_mixed_form_kernel(const int8_t *__restrict__ in, size_t station_pitch,
 }
 
 // n_freq above 16384: the transform staged through HBM (fft_body.hpp
-// staged_*). Stage 1: grid (ceil(P/256), m, frames * stations).
-constexpr unsigned kStagedBlock = 256;
-__global__ void __launch_bounds__(kStagedBlock)
-staged_first_kernel(const int8_t *__restrict__ in, size_t station_pitch, const int32_t *__restrict__ valid,
-                    size_t valid_stride, int frame0, int frames, int n, int m, int p,
-                    const float2 *__restrict__ tw, float2 *__restrict__ ys) {
+// staged_*). Stage 1: grid (m, frames, stations), P/16 lanes, the P-point
+// exchange buffer (139 KB at P = 16384).
+template <int LOG2P>
+__global__ void __launch_bounds__((1 << LOG2P) / 16)
+staged_first_sub_kernel(const int8_t *__restrict__ in, size_t station_pitch, const int32_t *__restrict__ valid,
+                        size_t valid_stride, int frame0, int frames, int n, int m, const float2 *__restrict__ tw,
+                        float2 *__restrict__ zs) {
+    __shared__ float2 lds[fft::Sizes<(1 << LOG2P)>::kLds];
+    DeviceSync sync;
     fft::StagedPlan pl;
     pl.n = n;
     pl.m = m;
-    pl.p = p;
-    const int f = (int)(blockIdx.z % (unsigned)frames), st = (int)(blockIdx.z / (unsigned)frames);
-    fft::staged_first((int)(blockIdx.x * kStagedBlock + threadIdx.x), (int)blockIdx.y, f, st, frames, pl, in,
-                      station_pitch, frame0, valid, valid_stride, tw, ys);
+    pl.p = 1 << LOG2P;
+    pl.log2p = LOG2P;
+    fft::staged_first_sub<LOG2P>((int)threadIdx.x, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, frames, pl, in,
+                                 station_pitch, frame0, valid, valid_stride, lds, sync, tw, tw + n, zs);
 }
 
-// Stage 2: grid (m, frames, stations), P/16 lanes, the P-point exchange buffer
-// (139 KB at P = 16384).
-template <int LOG2P>
-__global__ void __launch_bounds__((1 << LOG2P) / 16)
-staged_sub_kernel(const int32_t *__restrict__ valid, size_t valid_stride, int frame0, int frames, int m,
-                  const float2 *__restrict__ ys, const float2 *__restrict__ twp, float2 *__restrict__ zs) {
-    __shared__ float2 lds[fft::Sizes<(1 << LOG2P)>::kLds];
-    DeviceSync sync;
-    fft::staged_sub<LOG2P>((int)threadIdx.x, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, frames, m, frame0,
-                           valid, valid_stride, lds, sync, ys, twp, zs);
-}
-
-// Stage 3: grid (ceil((N/2+1)/256), frames, stations). Every form's words in
+// Stage 2: grid (ceil((N/2+1)/256), frames, stations). Every form's words in
 // the order of forms::spec_index (tile 16, 32 or N, the plain order): FORM 0
 // complex64, FORM 1/2 bin k scaled by gain[st*N + k] and requantized.
+constexpr unsigned kStagedBlock = 256;
 template <int FORM>
 __global__ void __launch_bounds__(kStagedBlock)
 staged_r2c_kernel(const int32_t *__restrict__ valid, size_t valid_stride, int frame0, int frames, int n, int m,
@@ -513,25 +505,20 @@ hipError_t launch_fengine_fp32(int n_freq, const int8_t *in, size_t station_pitc
 hipError_t launch_fengine_staged(int form, int n_freq, const int8_t *in, size_t station_pitch, const int32_t *valid,
                                  size_t valid_stride, int frame0, int frames, int n_station, const float2 *tw,
                                  const float2 *wl, const float *gain, void *out, size_t out_station_stride, int tile,
-                                 float2 *ys, float2 *zs, hipStream_t s) {
+                                 float2 *zs, hipStream_t s) {
     if (frames <= 0 || n_station <= 0) return hipSuccess;
     const fft::StagedPlan pl = fft::staged_plan(n_freq);
     if (pl.m == 0 || form < 0 || form > 2 || (tile != 16 && tile != 32 && tile != n_freq) ||
         (size_t)frames * (size_t)n_station > 65535u)
         return hipErrorInvalidValue;
-    hipLaunchKernelGGL(staged_first_kernel, dim3((unsigned)((pl.p + kStagedBlock - 1) / kStagedBlock), (unsigned)pl.m,
-                                                 (unsigned)(frames * n_station)),
-                       dim3(kStagedBlock), 0, s, in, station_pitch, valid, valid_stride, frame0, frames, pl.n, pl.m,
-                       pl.p, tw, ys);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
     const dim3 sub_grid((unsigned)pl.m, (unsigned)frames, (unsigned)n_station);
     const dim3 sub_block((unsigned)(pl.p / 16));
+    hipError_t e;
     switch (pl.log2p) {
 #define CORR_STAGED_CASE(L)                                                                                          \
     case L:                                                                                                           \
-        hipLaunchKernelGGL(staged_sub_kernel<L>, sub_grid, sub_block, 0, s, valid, valid_stride, frame0, frames, pl.m, \
-                           ys, tw + n_freq, zs);                                                                      \
+        hipLaunchKernelGGL(staged_first_sub_kernel<L>, sub_grid, sub_block, 0, s, in, station_pitch, valid,            \
+                           valid_stride, frame0, frames, pl.n, pl.m, tw, zs);                                        \
         break;
         CORR_STAGED_CASE(11)
         CORR_STAGED_CASE(12)

@@ -196,26 +196,20 @@ hipError_t launch_fengine_fp32(int n_freq, const int8_t *in, size_t station_pitc
 hipError_t launch_fengine_staged(int form, int n_freq, const int8_t *in, size_t station_pitch, const int32_t *valid,
                                  size_t valid_stride, int frame0, int frames, int n_station, const float2 *tw,
                                  const float2 *wl, const float *gain, void *out, size_t out_station_stride, int tile,
-                                 float2 *ys, float2 *zs, hipStream_t) {
+                                 float2 *zs, hipStream_t) {
     if (frames <= 0 || n_station <= 0) return hipSuccess;
     const fft::StagedPlan pl = fft::staged_plan(n_freq);
     if (pl.m == 0 || form < 0 || form > 2 || (tile != 16 && tile != 32 && tile != n_freq) ||
         (size_t)frames * (size_t)n_station > 65535u)
         return hipErrorInvalidValue;
-    // stage 1, every lane of the device grid (ceil(P/256) x 256 lanes per (k2, frame, station))
-    const int lanes1 = (pl.p + 255) / 256 * 256;
-    for (int st = 0; st < n_station; ++st)
-        for (int f = 0; f < frames; ++f)
-            for (int k2 = 0; k2 < pl.m; ++k2)
-                for (int n1 = 0; n1 < lanes1; ++n1)
-                    fft::staged_first(n1, k2, f, st, frames, pl, in, station_pitch, frame0, valid, valid_stride, tw, ys);
-    // stage 2, one workgroup of P/16 lanes per (k2, frame, station)
+    // stage 1, one workgroup of P/16 lanes per (k2, frame, station)
     auto sub = [&](auto log2p_tag) {
         constexpr int L = decltype(log2p_tag)::value;
         run_grid(pl.m, frames * n_station, (1 << L) / 16, fft::Sizes<(1 << L)>::kLds * sizeof(float2),
                  [=](int j, int bx, int by, char *lds, HostSync &sync) {
-                     fft::staged_sub<L>(j, bx, by % frames, by / frames, frames, pl.m, frame0, valid, valid_stride,
-                                        reinterpret_cast<float2 *>(lds), sync, ys, tw + n_freq, zs);
+                     fft::staged_first_sub<L>(j, bx, by % frames, by / frames, frames, pl, in, station_pitch, frame0,
+                                              valid, valid_stride, reinterpret_cast<float2 *>(lds), sync, tw,
+                                              tw + n_freq, zs);
                  });
     };
     switch (pl.log2p) {
@@ -225,13 +219,13 @@ hipError_t launch_fengine_staged(int form, int n_freq, const int8_t *in, size_t 
     case 14: sub(std::integral_constant<int, 14>{}); break;
     default: return hipErrorInvalidValue;
     }
-    // stage 3, every lane of the device grid
-    const int lanes3 = (n_freq / 2 + 1 + 255) / 256 * 256;
+    // stage 2, every lane of the device grid
+    const int lanes2 = (n_freq / 2 + 1 + 255) / 256 * 256;
     const int fcap = (int)(out_station_stride / (size_t)n_freq);
     for (int st = 0; st < n_station; ++st)
         for (int f = 0; f < frames; ++f) {
             const int frame = frame0 + f;
-            for (int k = 0; k < lanes3; ++k) {
+            for (int k = 0; k < lanes2; ++k) {
                 if (form == 0) {
                     float2 *o = static_cast<float2 *>(out);
                     auto at = [&](int b) { return forms::spec_index(st, frame, b, n_freq, fcap, tile); };

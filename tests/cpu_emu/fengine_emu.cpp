@@ -368,12 +368,7 @@ static int check_staged(int n) {
     }
     for (int m = 0; m < pl.p; ++m)
         tw[n + m] = make_float2((float)std::cos(-2.0 * M_PI * m / pl.p), (float)std::sin(-2.0 * M_PI * m / pl.p));
-    std::vector<float2> ys((size_t)NST * FR * n), zs((size_t)NST * FR * n), out((size_t)NST * FR * n);
-    for (int st = 0; st < NST; ++st)
-        for (int f = 0; f < FR; ++f)
-            for (int k2 = 0; k2 < pl.m; ++k2)
-                for (int n1 = 0; n1 < (pl.p + 255) / 256 * 256; ++n1)
-                    staged_first(n1, k2, f, st, FR, pl, in.data(), pitch, 0, valid.data(), FR, tw.data(), ys.data());
+    std::vector<float2> zs((size_t)NST * FR * n), out((size_t)NST * FR * n);
     auto sub = [&](auto tag) {
         constexpr int L = decltype(tag)::value;
         for (int st = 0; st < NST; ++st)
@@ -385,8 +380,8 @@ static int check_staged(int n) {
                     for (int j = 0; j < (1 << L) / 16; ++j)
                         th.emplace_back([&, j, st, f, k2] {
                             HostSync sync{&bar};
-                            staged_sub<L>(j, k2, f, st, FR, pl.m, 0, valid.data(), FR, lds, sync, ys.data(),
-                                          tw.data() + n, zs.data());
+                            staged_first_sub<L>(j, k2, f, st, FR, pl, in.data(), pitch, 0, valid.data(), FR, lds,
+                                                sync, tw.data(), tw.data() + n, zs.data());
                         });
                     for (auto &t : th) t.join();
                     delete[] lds;
@@ -458,6 +453,7 @@ int main(int argc, char **argv) {
     if (mine()) bad += check_staged(32768);
     if (mine()) bad += check_staged(45056);
     if (mine()) bad += check_staged(67584);
+    if (mine()) bad += check_staged(40960);  // m = 5
     std::printf(bad ? "FAILED\n" : "OK\n");
     return bad ? 1 : 0;
 }
