@@ -540,8 +540,10 @@ corr_status Engine::transform(int form, const int8_t *d_in, size_t station_pitch
                               hipStream_t s) {
     const int N = shape_.n_station, nf = shape_.n_freq;
     if (d_stage_ != nullptr) {
-        for (int f0 = 0; f0 < frames; f0 += stage_frames_) {
-            const int n = frames - f0 < stage_frames_ ? frames - f0 : stage_frames_;
+        // the one-kernel form stages nothing: its launches take every frame
+        const int batch = fft::staged_one_kernel(form, fft::staged_plan(nf)) ? 65535 : stage_frames_;
+        for (int f0 = 0; f0 < frames; f0 += batch) {
+            const int n = frames - f0 < batch ? frames - f0 : batch;
             corr_status st = hip_check(launch_fengine_staged(form, nf, d_in, station_pitch, valid, valid_stride, f0, n,
                                                              N, d_tw_, d_wl_, d_gain_, out, out_station_stride, tile,
                                                              d_stage_, s),

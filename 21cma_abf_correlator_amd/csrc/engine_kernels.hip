@@ -143,8 +143,7 @@ fengine_mixed_form_kernel(const int8_t *__restrict__ in, size_t station_pitch,
                                                   out_station_stride, tile);
 }
 
-// n_freq above 16384: the transform staged through HBM (fft_body.hpp
-// staged_*). Stage 1: grid (m, frames, stations), P/16 lanes, the P-point
+// n_freq above 16384, the two-kernel form (fft_body.hpp staged_*). Stage 1: grid (m, frames, stations), P/16 lanes, the P-point
 // exchange buffer (139 KB at P = 16384).
 template <int LOG2P>
 __global__ void __launch_bounds__((1 << LOG2P) / 16)
@@ -196,6 +195,45 @@ staged_r2c_kernel(const int32_t *__restrict__ valid, size_t valid_stride, int fr
         fft::staged_r2c((int)(blockIdx.x * kStagedBlock + threadIdx.x), f, st, frames, pl, frame0, valid,
                         valid_stride, zs, wl, [&](int k, float2 x) { o[at(k)] = forms::pack_int8(x, g[k]); },
                         [&](int k) { o[at(k)] = (uint16_t)0; });
+    }
+}
+
+// n_freq above 16384, the one-kernel form (fft_body.hpp staged_frame, used
+// where staged_one_kernel holds): grid (m/2 + 1, frames, stations), P/16
+// lanes, the P-point exchange buffer (139 KB at P = 16384). The FP16 (FORM 1)
+// and int8 (FORM 2) forms' words, bin k scaled by gain[st*N + k] and
+// requantized, in the order of forms::spec_index (tile 16, 32 or N).
+template <int LOG2P, int FORM>
+__global__ void __launch_bounds__((1 << LOG2P) / 16)
+staged_kernel(const int8_t *__restrict__ in, size_t station_pitch, const int32_t *__restrict__ valid,
+              size_t valid_stride, int frame0, int n, int m, const float2 *__restrict__ tw,
+              const float2 *__restrict__ wl, const float *__restrict__ gain, void *__restrict__ out,
+              size_t out_station_stride, int tile) {
+    __shared__ float2 lds[fft::Sizes<(1 << LOG2P)>::kLds];
+    DeviceSync sync;
+    fft::StagedPlan pl;
+    pl.n = n;
+    pl.m = m;
+    pl.p = 1 << LOG2P;
+    pl.log2p = LOG2P;
+    const int q = (int)blockIdx.x, f = (int)blockIdx.y, st = (int)blockIdx.z;
+    const int frame = frame0 + f;
+    const int fcap = (int)(out_station_stride / (size_t)n);
+    auto at = [&](int k) { return forms::spec_index(st, frame, k, n, fcap, tile); };
+    const int j = (int)threadIdx.x;
+    static_assert(FORM == 1 || FORM == 2, "the one-kernel form writes the requantized forms");
+    if constexpr (FORM == 1) {
+        uint32_t *o = static_cast<uint32_t *>(out);
+        const float *g = gain + (size_t)st * (size_t)n;
+        fft::staged_frame<LOG2P>(j, q, f, st, pl, in, station_pitch, frame0, valid, valid_stride, lds, sync, tw,
+                                 tw + n, wl, [&](int k, float2 x) { o[at(k)] = forms::pack_fp16(x, g[k]); },
+                                 [&](int k) { o[at(k)] = 0u; });
+    } else {
+        uint16_t *o = static_cast<uint16_t *>(out);
+        const float *g = gain + (size_t)st * (size_t)n;
+        fft::staged_frame<LOG2P>(j, q, f, st, pl, in, station_pitch, frame0, valid, valid_stride, lds, sync, tw,
+                                 tw + n, wl, [&](int k, float2 x) { o[at(k)] = forms::pack_int8(x, g[k]); },
+                                 [&](int k) { o[at(k)] = (uint16_t)0; });
     }
 }
 
@@ -508,9 +546,20 @@ hipError_t launch_fengine_staged(int form, int n_freq, const int8_t *in, size_t 
                                  float2 *zs, hipStream_t s) {
     if (frames <= 0 || n_station <= 0) return hipSuccess;
     const fft::StagedPlan pl = fft::staged_plan(n_freq);
-    if (pl.m == 0 || form < 0 || form > 2 || (tile != 16 && tile != 32 && tile != n_freq) ||
-        (size_t)frames * (size_t)n_station > 65535u)
+    if (pl.m == 0 || form < 0 || form > 2 || (tile != 16 && tile != 32 && tile != n_freq) || frames > 65535 ||
+        n_station > 65535)
         return hipErrorInvalidValue;
+    if (fft::staged_one_kernel(form, pl)) {
+        const dim3 grid((unsigned)(pl.m / 2 + 1), (unsigned)frames, (unsigned)n_station);
+        if (form == 1)
+            hipLaunchKernelGGL((staged_kernel<14, 1>), grid, dim3(1024), 0, s, in, station_pitch, valid, valid_stride,
+                               frame0, pl.n, pl.m, tw, wl, gain, out, out_station_stride, tile);
+        else
+            hipLaunchKernelGGL((staged_kernel<14, 2>), grid, dim3(1024), 0, s, in, station_pitch, valid, valid_stride,
+                               frame0, pl.n, pl.m, tw, wl, gain, out, out_station_stride, tile);
+        return hipGetLastError();
+    }
+    if ((size_t)frames * (size_t)n_station > 65535u) return hipErrorInvalidValue;  // the two-kernel grids
     const dim3 sub_grid((unsigned)pl.m, (unsigned)frames, (unsigned)n_station);
     const dim3 sub_block((unsigned)(pl.p / 16));
     hipError_t e;

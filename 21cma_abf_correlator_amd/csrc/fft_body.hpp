@@ -1426,19 +1426,20 @@ CORR_HD CORR_FI void fengine_mixed_form_block(int j, int f, int st, float2 *lds,
 // ---------------------------------------------------------------------------
 // n_freq above 16384 (every multiple of 2048 up to 2^20; the reference accepts
 // every multiple of 2048, corr-cuda/src/corr_cuda.cpp:500-509): one frame's
-// transform no longer fits a workgroup's LDS, so it is staged through HBM in
-// two kernels, N = m P with P the largest power of two dividing N, at most
-// 16384, and m = N / P (2 to 511; the radix-m stage costs m complex products
-// per point):
-//   1. one workgroup per sub-sequence k2 forms, from the int8 input,
-//        y_k2[n1] = W_N^(n1 k2) sum_n2 z[n1 + P n2] W_m^(n2 k2)
-//      with W_m^r = W_N^(r P), the phase n2 k2 reduced mod m exactly, and
-//      runs its P-point transform, the Stockham passes above in LDS:
-//      Z[m k1 + k2] = DFT_P(y_k2)[k1], stored as Zs[k2][k1];
-//   2. the R2C unpack of bins k and N-k from Z, one lane per pair, written in
-//      the form's representation and spectrum order.
-// The staging buffer holds [station][frame][m][P] complex64 of one batch of
-// frames. A missing frame skips stage 1 and is written as zeros by 2.
+// transform no longer fits a workgroup's LDS: N = m P with P the largest power
+// of two dividing N, at most 16384, and m = N / P (2 to 511), and
+//   Z[m k1 + k2] = DFT_P(y_k2)[k1],
+//   y_k2[n1] = W_N^(n1 k2) sum_n2 z[n1 + P n2] W_m^(n2 k2)
+// with W_m^r = W_N^(r P), the phase n2 k2 reduced mod m exactly (m complex
+// products per point). Two forms, both from staged_sub_transform:
+//  - one kernel (staged_frame): a workgroup transforms the sub-sequences of two
+//    residues k2 that mirror each other under k -> N - k, one after the other
+//    in LDS, and writes their R2C bins; nothing is staged, but the bins of one
+//    workgroup lie m apart, so its stores scatter as m grows;
+//  - two kernels: a workgroup per sub-sequence stores Z_k2 to a staging buffer
+//    ([station][frame][m][P] complex64 of a batch of frames), and a lane per
+//    bin pair reads Z[k] and Z[N-k] back and writes the bins in order.
+// A missing frame's bins are written as zeros.
 // ---------------------------------------------------------------------------
 struct StagedPlan {
     int n = 0, m = 0, p = 0, log2p = 0;
@@ -1460,23 +1461,23 @@ CORR_HD inline StagedPlan staged_plan(int n_freq) {
     return s;
 }
 
-// Stage 1, lane j of the workgroup of sub-sequence k2 of frame f of station st
-// (P/16 lanes): y_k2 from the frame's int8 samples (each of the frame's m
-// workgroups reads all of them), then its P-point transform into zs at
-// ((st * frames + f) * m + k2) * P. Forming y_k2 here, and not in a kernel of
-// its own through a second staging buffer, saves a round trip of 8 bytes per
-// point through HBM and halved the transform's time: 19.4 -> 11.0 ms per 2048
-// frames of 24 inputs at n_freq 32768, 14.6 -> 7.4 ms at 18432, 110 -> 44 ms
-// at 67584.
+// The one-kernel form where it measured faster than the two-kernel one: the
+// FP16 and int8 forms (4 and 2 bytes a bin) at P = 16384 and m <= 4, where a
+// wave's stores span at most 1 KB. The FP32 form's 8-byte bins, and larger m,
+// scatter further and measured slower than the staging round trip.
+CORR_HD inline bool staged_one_kernel(int form, const StagedPlan &pl) {
+    return form != 0 && pl.p == 16384 && pl.m <= 4;
+}
+
+// The P-point transform of sub-sequence k2 of the frame whose samples start at
+// z, lane j of P/16: for its points n1 = j + r P/16,
+//   y_k2[n1] = W_N^(n1 k2) sum_n2 z[n1 + P n2] W_m^(n2 k2),
+// then the Stockham passes. On return lds holds Z_k2[k1] = Z[m k1 + k2] at k1
+// (plain layout) after a final barrier.
 template <int LOG2P, class Sync>
-CORR_HD CORR_FI void staged_first_sub(int j, int k2, int f, int st, int frames, const StagedPlan &pl,
-                                     const int8_t *in, size_t station_pitch, int frame0, const int32_t *valid,
-                                     size_t valid_stride, float2 *lds, Sync &sync, const float2 *twn,
-                                     const float2 *twp, float2 *zs) {
+CORR_HD CORR_FI void staged_sub_transform(int j, int k2, const char2 *z, const StagedPlan &pl, float2 *lds,
+                                         Sync &sync, const float2 *twn, const float2 *twp) {
     constexpr int P = 1 << LOG2P, NT = P / 16;
-    if (!frame_present(valid, valid_stride, frame0 + f, st)) return;  // uniform over the workgroup
-    const char2 *z = reinterpret_cast<const char2 *>(in + (size_t)st * station_pitch +
-                                                     (size_t)(frame0 + f) * (size_t)(2 * pl.n));
     float2 acc[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[r] = make_float2(0.0f, 0.0f);
@@ -1498,12 +1499,27 @@ CORR_HD CORR_FI void staged_first_sub(int j, int k2, int f, int st, int frames, 
 #pragma unroll
     for (int r = 0; r < 16; ++r) v[r] = cmul(acc[r], twn[(unsigned)((j + r * NT) * k2)]);
     fft_from_values<LOG2P>(j, v, lds, sync, twp);
+}
+
+// Two-kernel form, stage 1: the workgroup of sub-sequence k2 of frame f of
+// station st transforms it and stores Z_k2 to zs at ((st * frames + f) * m +
+// k2) * P (the staging buffer of one batch of frames).
+template <int LOG2P, class Sync>
+CORR_HD CORR_FI void staged_first_sub(int j, int k2, int f, int st, int frames, const StagedPlan &pl,
+                                     const int8_t *in, size_t station_pitch, int frame0, const int32_t *valid,
+                                     size_t valid_stride, float2 *lds, Sync &sync, const float2 *twn,
+                                     const float2 *twp, float2 *zs) {
+    constexpr int P = 1 << LOG2P, NT = P / 16;
+    if (!frame_present(valid, valid_stride, frame0 + f, st)) return;  // uniform over the workgroup
+    const char2 *z = reinterpret_cast<const char2 *>(in + (size_t)st * station_pitch +
+                                                     (size_t)(frame0 + f) * (size_t)(2 * pl.n));
+    staged_sub_transform<LOG2P>(j, k2, z, pl, lds, sync, twn, twp);
     const size_t base = (((size_t)st * frames + f) * pl.m + k2) * P;
 #pragma unroll
     for (int r = 0; r < 16; ++r) zs[base + (size_t)(j + r * NT)] = lds[j + r * NT];
 }
 
-// Stage 2, lane k (0 <= k <= N/2) of frame f of station st: bins k and N-k
+// Two-kernel form, stage 2: lane k (0 <= k <= N/2) of frame f of station st: bins k and N-k
 // (k alone for 0 and N/2) from Z[k] = zs[k mod m][k div m]. `put(k, X)`
 // stores bin k, `zero(k)` a missing frame's bin.
 template <class Put, class Zero>
@@ -1526,6 +1542,64 @@ CORR_HD CORR_FI void staged_r2c(int k, int f, int st, int frames, const StagedPl
     r2c_from(a, b, wl[(unsigned)k], xk, xm);
     put(k, xk);
     if (pair) put(n - k, xm);
+}
+
+// One-kernel form: workgroup q (0 <= q <= m/2) of frame f of station st, lane j of P/16: every
+// bin k = m k1 + c of the residues c = a = q and c = b = (m - q) mod m, written
+// by put(k, X) (zero(k) for a missing frame). Bin k needs Z[k] and Z[N-k], and
+// N - (m k1 + a) = m (P-1-k1) + b for a > 0, so the workgroup transforms
+// sub-sequence a, keeps its spectrum in registers (16 points a lane), transforms
+// b in the same LDS and forms each pair (k, N-k) from the two. For a = b (the
+// residue 0, and m/2 for even m) one transform serves, each mirror pair formed
+// once; bins 0 and N/2 (residue 0, k1 = 0 and P/2) are alone.
+// Each of the frame's m/2 + 1 workgroups reads all of the frame's samples (2N
+// bytes, m times the input in all, from L2 for the most part); nothing but the
+// spectra is written.
+template <int LOG2P, class Sync, class Put, class Zero>
+CORR_HD CORR_FI void staged_frame(int j, int q, int f, int st, const StagedPlan &pl, const int8_t *in,
+                                 size_t station_pitch, int frame0, const int32_t *valid, size_t valid_stride,
+                                 float2 *lds, Sync &sync, const float2 *twn, const float2 *twp, const float2 *wl,
+                                 Put &&put, Zero &&zero) {
+    constexpr int P = 1 << LOG2P, NT = P / 16;
+    const int m = pl.m, n = pl.n, a = q, b = q == 0 ? 0 : m - q;
+    if (!frame_present(valid, valid_stride, frame0 + f, st)) {  // uniform over the workgroup
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int k1 = j + r * NT;
+            zero(m * k1 + a);
+            if (b != a) zero(m * k1 + b);
+        }
+        return;
+    }
+    const char2 *z = reinterpret_cast<const char2 *>(in + (size_t)st * station_pitch +
+                                                     (size_t)(frame0 + f) * (size_t)(2 * n));
+    staged_sub_transform<LOG2P>(j, a, z, pl, lds, sync, twn, twp);
+    if (b != a) {
+        float2 za[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) za[r] = lds[j + r * NT];
+        sync();  // every lane holds its points of Z_a before Z_b overwrites them
+        staged_sub_transform<LOG2P>(j, b, z, pl, lds, sync, twn, twp);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int k1 = j + r * NT, k = m * k1 + a;
+            float2 xk, xm;
+            r2c_from(za[r], lds[P - 1 - k1], wl[(unsigned)k], xk, xm);
+            put(k, xk);
+            put(n - k, xm);
+        }
+        return;
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int k1 = j + r * NT;
+        if (a == 0 ? k1 > P / 2 : k1 >= P / 2) continue;
+        const int k1m = a == 0 ? (P - k1) & (P - 1) : P - 1 - k1, k = m * k1 + a;
+        float2 xk, xm;
+        r2c_from(lds[k1], lds[k1m], wl[(unsigned)k], xk, xm);
+        put(k, xk);
+        if (k1m != k1) put(n - k, xm);
+    }
 }
 
 // Calibration: power[st*nf + c] += sum over the frames present of |Y|^2, in

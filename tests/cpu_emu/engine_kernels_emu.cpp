@@ -199,9 +199,44 @@ hipError_t launch_fengine_staged(int form, int n_freq, const int8_t *in, size_t 
                                  float2 *zs, hipStream_t) {
     if (frames <= 0 || n_station <= 0) return hipSuccess;
     const fft::StagedPlan pl = fft::staged_plan(n_freq);
-    if (pl.m == 0 || form < 0 || form > 2 || (tile != 16 && tile != 32 && tile != n_freq) ||
-        (size_t)frames * (size_t)n_station > 65535u)
+    if (pl.m == 0 || form < 0 || form > 2 || (tile != 16 && tile != 32 && tile != n_freq) || frames > 65535 ||
+        n_station > 65535)
         return hipErrorInvalidValue;
+    if (fft::staged_one_kernel(form, pl)) {
+    const int fcap = (int)(out_station_stride / (size_t)n_freq);
+        // one workgroup of P/16 lanes per (residue pair, frame, station), as the device grid
+        auto run = [&](auto log2p_tag) {
+            constexpr int L = decltype(log2p_tag)::value;
+            run_grid(pl.m / 2 + 1, frames * n_station, (1 << L) / 16, fft::Sizes<(1 << L)>::kLds * sizeof(float2),
+                     [=](int j, int bx, int by, char *lds_bytes, HostSync &sync) {
+                         const int f = by % frames, st = by / frames, frame = frame0 + f;
+                         float2 *lds = reinterpret_cast<float2 *>(lds_bytes);
+                         auto at = [&](int k) { return forms::spec_index(st, frame, k, n_freq, fcap, tile); };
+                         const float *g = gain == nullptr ? nullptr : gain + (size_t)st * n_freq;
+                         if (form == 0) {
+                             float2 *o = static_cast<float2 *>(out);
+                             fft::staged_frame<L>(j, bx, f, st, pl, in, station_pitch, frame0, valid, valid_stride, lds,
+                                                  sync, tw, tw + n_freq, wl, [&](int k, float2 x) { o[at(k)] = x; },
+                                                  [&](int k) { o[at(k)] = make_float2(0.0f, 0.0f); });
+                         } else if (form == 1) {
+                             uint32_t *o = static_cast<uint32_t *>(out);
+                             fft::staged_frame<L>(j, bx, f, st, pl, in, station_pitch, frame0, valid, valid_stride, lds,
+                                                  sync, tw, tw + n_freq, wl,
+                                                  [&](int k, float2 x) { o[at(k)] = forms::pack_fp16(x, g[k]); },
+                                                  [&](int k) { o[at(k)] = 0u; });
+                         } else {
+                             uint16_t *o = static_cast<uint16_t *>(out);
+                             fft::staged_frame<L>(j, bx, f, st, pl, in, station_pitch, frame0, valid, valid_stride, lds,
+                                                  sync, tw, tw + n_freq, wl,
+                                                  [&](int k, float2 x) { o[at(k)] = forms::pack_int8(x, g[k]); },
+                                                  [&](int k) { o[at(k)] = (uint16_t)0; });
+                         }
+                     });
+        };
+        run(std::integral_constant<int, 14>{});
+        return hipSuccess;
+    }
+    if ((size_t)frames * (size_t)n_station > 65535u) return hipErrorInvalidValue;  // the two-kernel grids
     // stage 1, one workgroup of P/16 lanes per (k2, frame, station)
     auto sub = [&](auto log2p_tag) {
         constexpr int L = decltype(log2p_tag)::value;

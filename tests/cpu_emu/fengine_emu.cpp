@@ -349,9 +349,11 @@ static int check_unpack4() {
     return bad;
 }
 
-// n_freq above 16384: the three staged bodies in launch order, FP32 against
-// the oracle's FP64 FFT (any length), one missing frame written as zeros.
-static int check_staged(int n) {
+// n_freq above 16384: the two-kernel form (its bodies in launch order) or the
+// one-kernel form over the device grid, FP32 against the oracle's FP64 FFT (any
+// length), one missing frame written as zeros; every bin starts as NaN, so one
+// that no lane writes fails.
+static int check_staged(int n, bool one_kernel) {
     const StagedPlan pl = staged_plan(n);
     const int NST = 2, FR = 2;
     const size_t pitch = (size_t)FR * 2 * n + 16;
@@ -369,6 +371,27 @@ static int check_staged(int n) {
     for (int m = 0; m < pl.p; ++m)
         tw[n + m] = make_float2((float)std::cos(-2.0 * M_PI * m / pl.p), (float)std::sin(-2.0 * M_PI * m / pl.p));
     std::vector<float2> zs((size_t)NST * FR * n), out((size_t)NST * FR * n);
+    auto run = [&](auto tag) {
+        constexpr int L = decltype(tag)::value;
+        for (int st = 0; st < NST; ++st)
+            for (int f = 0; f < FR; ++f)
+                for (int q = 0; q <= pl.m / 2; ++q) {
+                    float2 *lds = new float2[Sizes<(1 << L)>::kLds];
+                    std::barrier<> bar((1 << L) / 16);
+                    std::vector<std::thread> th;
+                    float2 *o = out.data() + ((size_t)st * FR + f) * n;
+                    for (int j = 0; j < (1 << L) / 16; ++j)
+                        th.emplace_back([&, j, st, f, q, o] {
+                            HostSync sync{&bar};
+                            staged_frame<L>(j, q, f, st, pl, in.data(), pitch, 0, valid.data(), FR, lds, sync,
+                                            tw.data(), tw.data() + n, wl.data(), [&](int b, float2 x) { o[b] = x; },
+                                            [&](int b) { o[b] = make_float2(0.0f, 0.0f); });
+                        });
+                    for (auto &t : th) t.join();
+                    delete[] lds;
+                }
+    };
+    std::fill(out.begin(), out.end(), make_float2(std::nanf(""), 0.0f));  // every bin must be written
     auto sub = [&](auto tag) {
         constexpr int L = decltype(tag)::value;
         for (int st = 0; st < NST; ++st)
@@ -387,17 +410,23 @@ static int check_staged(int n) {
                     delete[] lds;
                 }
     };
-    if (pl.log2p == 11) sub(std::integral_constant<int, 11>{});
-    else if (pl.log2p == 12) sub(std::integral_constant<int, 12>{});
-    else if (pl.log2p == 13) sub(std::integral_constant<int, 13>{});
-    else sub(std::integral_constant<int, 14>{});
-    for (int st = 0; st < NST; ++st)
-        for (int f = 0; f < FR; ++f) {
-            float2 *o = out.data() + ((size_t)st * FR + f) * n;
-            for (int k = 0; k < (n / 2 + 1 + 255) / 256 * 256; ++k)
-                staged_r2c(k, f, st, FR, pl, 0, valid.data(), FR, zs.data(), wl.data(),
-                           [&](int b, float2 x) { o[b] = x; }, [&](int b) { o[b] = make_float2(0.0f, 0.0f); });
-        }
+    if (!one_kernel) {
+        if (pl.log2p == 11) sub(std::integral_constant<int, 11>{});
+        else if (pl.log2p == 12) sub(std::integral_constant<int, 12>{});
+        else if (pl.log2p == 13) sub(std::integral_constant<int, 13>{});
+        else sub(std::integral_constant<int, 14>{});
+        for (int st = 0; st < NST; ++st)
+            for (int f = 0; f < FR; ++f) {
+                float2 *o = out.data() + ((size_t)st * FR + f) * n;
+                for (int k = 0; k < (n / 2 + 1 + 255) / 256 * 256; ++k)
+                    staged_r2c(k, f, st, FR, pl, 0, valid.data(), FR, zs.data(), wl.data(),
+                               [&](int b, float2 x) { o[b] = x; }, [&](int b) { o[b] = make_float2(0.0f, 0.0f); });
+            }
+    }
+    else if (pl.log2p == 11) run(std::integral_constant<int, 11>{});
+    else if (pl.log2p == 12) run(std::integral_constant<int, 12>{});
+    else if (pl.log2p == 13) run(std::integral_constant<int, 13>{});
+    else run(std::integral_constant<int, 14>{});
     int bad = 0;
     double worst = 0.0;
     std::vector<double> x(2 * (size_t)n), ref(2 * (size_t)n);
@@ -421,7 +450,8 @@ static int check_staged(int n) {
             if (e > worst) worst = e;
             if (!(e < 1e-5)) ++bad;
         }
-    std::printf("N=%d staged (%d x %d): worst relative RMS %.3e, %d bad\n", n, pl.m, pl.p, worst, bad);
+    std::printf("N=%d staged (%d x %d, %s): worst relative RMS %.3e, %d bad\n", n, pl.m, pl.p,
+                one_kernel ? "one kernel" : "two kernels", worst, bad);
     return bad;
 }
 
@@ -449,11 +479,18 @@ int main(int argc, char **argv) {
     if (mine()) bad += check_mixed<3, 12>();
     if (mine()) bad += check_mixed<7, 11>();
     if (mine()) bad += check_unpack4();
-    if (mine()) bad += check_staged(18432);
-    if (mine()) bad += check_staged(32768);
-    if (mine()) bad += check_staged(45056);
-    if (mine()) bad += check_staged(67584);
-    if (mine()) bad += check_staged(40960);  // m = 5
+    if (mine()) bad += check_staged(18432, false);
+    if (mine()) bad += check_staged(32768, false);
+    if (mine()) bad += check_staged(45056, false);
+    if (mine()) bad += check_staged(67584, false);
+    if (mine()) bad += check_staged(40960, false);  // m = 5
+    // the one-kernel form: where the engine uses it (m = 2, 3, 4 at P = 16384),
+    // and at small P and odd m, where only this check runs it
+    if (mine()) bad += check_staged(32768, true);
+    if (mine()) bad += check_staged(49152, true);  // m = 3: a mirror pair and residue 0
+    if (mine()) bad += check_staged(65536, true);  // m = 4: a mirror pair and two residues alone
+    if (mine()) bad += check_staged(18432, true);
+    if (mine()) bad += check_staged(67584, true);
     std::printf(bad ? "FAILED\n" : "OK\n");
     return bad ? 1 : 0;
 }

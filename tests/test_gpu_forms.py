@@ -404,14 +404,17 @@ def test_c3_shape_fp16_error(corr, oracle, hip):
 
 
 @pytest.mark.parametrize("form,nf", [(1, 2048), (1, 16384), (2, 2048), (2, 16384), (1, 12288), (2, 14336),
-                                     (1, 18432), (2, 24576)])
+                                     (1, 18432), (2, 24576), (1, 32768), (1, 49152), (2, 65536)])
 def test_fengine_form_rounding(corr, oracle, hip, form, nf):
     """The form's F-engine output is the FP32 form's spectrum y requantized on
     the host: FP16 = y*g rounded to nearest even (numpy float16), int8 =
     clamp(rint(y*g), +-7), for at least 99.9% of values and every value within
     one FP16 ulp / one integer step (the two kernels share the FFT, so the rare
     difference can only come from a 1-ulp FP32 difference at a rounding tie).
-    Pins the rounding mode of the device's FP16 conversion."""
+    Pins the rounding mode of the device's FP16 conversion. At 32768, 49152
+    and 65536 (m = 2, 3, 4 sub-transforms of 16384) the forms' spectra come
+    from the one-kernel staged transform and the FP32 form's from the
+    two-kernel one."""
     N, T = 4, 8
     print(f"F-engine form {form} rounding N={N} nf={nf}", flush=True)
     x = oracle.gen_int8(21 + form, N, T, 2 * nf, 0.3, sigma=16.0, max_delay=4)
