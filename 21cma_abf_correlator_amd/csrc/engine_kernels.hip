@@ -199,16 +199,16 @@ staged_r2c_kernel(const int32_t *__restrict__ valid, size_t valid_stride, int fr
 }
 
 // n_freq above 16384, the one-kernel form (fft_body.hpp staged_frame, used
-// where staged_one_kernel holds): grid (m/2 + 1, frames, stations), P/16
-// lanes, the P-point exchange buffer (139 KB at P = 16384). The FP16 (FORM 1)
+// where staged_one_kernel holds): grid (m/2 + 1) * frames * stations, one
+// dimension, mapped by fft::staged_unit; P/16 lanes, the P-point exchange buffer (139 KB at P = 16384). The FP16 (FORM 1)
 // and int8 (FORM 2) forms' words, bin k scaled by gain[st*N + k] and
 // requantized, in the order of forms::spec_index (tile 16, 32 or N).
 template <int LOG2P, int FORM>
 __global__ void __launch_bounds__((1 << LOG2P) / 16)
 staged_kernel(const int8_t *__restrict__ in, size_t station_pitch, const int32_t *__restrict__ valid,
-              size_t valid_stride, int frame0, int n, int m, const float2 *__restrict__ tw,
-              const float2 *__restrict__ wl, const float *__restrict__ gain, void *__restrict__ out,
-              size_t out_station_stride, int tile) {
+              size_t valid_stride, int frame0, int frames, int n_station, int n, int m,
+              const float2 *__restrict__ tw, const float2 *__restrict__ wl, const float *__restrict__ gain,
+              void *__restrict__ out, size_t out_station_stride, int tile) {
     __shared__ float2 lds[fft::Sizes<(1 << LOG2P)>::kLds];
     DeviceSync sync;
     fft::StagedPlan pl;
@@ -216,7 +216,9 @@ staged_kernel(const int8_t *__restrict__ in, size_t station_pitch, const int32_t
     pl.m = m;
     pl.p = 1 << LOG2P;
     pl.log2p = LOG2P;
-    const int q = (int)blockIdx.x, f = (int)blockIdx.y, st = (int)blockIdx.z;
+    unsigned qu, u;
+    fft::staged_unit(blockIdx.x, (unsigned)(m / 2 + 1), (unsigned)frames * (unsigned)n_station, qu, u);
+    const int q = (int)qu, f = (int)(u % (unsigned)frames), st = (int)(u / (unsigned)frames);
     const int frame = frame0 + f;
     const int fcap = (int)(out_station_stride / (size_t)n);
     auto at = [&](int k) { return forms::spec_index(st, frame, k, n, fcap, tile); };
@@ -550,13 +552,15 @@ hipError_t launch_fengine_staged(int form, int n_freq, const int8_t *in, size_t 
         n_station > 65535)
         return hipErrorInvalidValue;
     if (fft::staged_one_kernel(form, pl)) {
-        const dim3 grid((unsigned)(pl.m / 2 + 1), (unsigned)frames, (unsigned)n_station);
+        const size_t groups = (size_t)(pl.m / 2 + 1) * (size_t)frames * (size_t)n_station;
+        if (groups > 0x7fffffffu) return hipErrorInvalidValue;
+        const dim3 grid((unsigned)groups);
         if (form == 1)
             hipLaunchKernelGGL((staged_kernel<14, 1>), grid, dim3(1024), 0, s, in, station_pitch, valid, valid_stride,
-                               frame0, pl.n, pl.m, tw, wl, gain, out, out_station_stride, tile);
+                               frame0, frames, n_station, pl.n, pl.m, tw, wl, gain, out, out_station_stride, tile);
         else
             hipLaunchKernelGGL((staged_kernel<14, 2>), grid, dim3(1024), 0, s, in, station_pitch, valid, valid_stride,
-                               frame0, pl.n, pl.m, tw, wl, gain, out, out_station_stride, tile);
+                               frame0, frames, n_station, pl.n, pl.m, tw, wl, gain, out, out_station_stride, tile);
         return hipGetLastError();
     }
     if ((size_t)frames * (size_t)n_station > 65535u) return hipErrorInvalidValue;  // the two-kernel grids

@@ -1469,6 +1469,25 @@ CORR_HD inline bool staged_one_kernel(int form, const StagedPlan &pl) {
     return form != 0 && pl.p == 16384 && pl.m <= 4;
 }
 
+// Workgroup id -> (residue pair q, unit u = st * frames + f) on the one-kernel
+// form's one-dimensional grid of X = m/2 + 1 workgroups for each of U units.
+// The X workgroups of a unit get ids 8 apart: dispatched round robin over the
+// 8 XCDs they share one XCD's L2, where their bins (interleaved m apart) merge
+// into whole lines before being written back. The last U mod 8 units take the
+// ids left, X in a row. A bijection of [0, X U) onto [0, X) x [0, U).
+CORR_HD CORR_FI void staged_unit(unsigned id, unsigned X, unsigned U, unsigned &q, unsigned &u) {
+    const unsigned full = U / 8u * 8u * X;
+    if (id < full) {
+        const unsigned b = id / (8u * X), r = id - b * 8u * X;
+        q = r / 8u;
+        u = b * 8u + r % 8u;
+    } else {
+        const unsigned r = id - full;
+        q = r % X;
+        u = U / 8u * 8u + r / X;
+    }
+}
+
 // The P-point transform of sub-sequence k2 of the frame whose samples start at
 // z, lane j of P/16: for its points n1 = j + r P/16,
 //   y_k2[n1] = W_N^(n1 k2) sum_n2 z[n1 + P n2] W_m^(n2 k2),

@@ -203,13 +203,17 @@ hipError_t launch_fengine_staged(int form, int n_freq, const int8_t *in, size_t 
         n_station > 65535)
         return hipErrorInvalidValue;
     if (fft::staged_one_kernel(form, pl)) {
-    const int fcap = (int)(out_station_stride / (size_t)n_freq);
-        // one workgroup of P/16 lanes per (residue pair, frame, station), as the device grid
+        const int fcap = (int)(out_station_stride / (size_t)n_freq);
+        // the device's one-dimensional grid, mapped by fft::staged_unit
+        const unsigned X = (unsigned)(pl.m / 2 + 1), U = (unsigned)frames * (unsigned)n_station;
         auto run = [&](auto log2p_tag) {
             constexpr int L = decltype(log2p_tag)::value;
-            run_grid(pl.m / 2 + 1, frames * n_station, (1 << L) / 16, fft::Sizes<(1 << L)>::kLds * sizeof(float2),
-                     [=](int j, int bx, int by, char *lds_bytes, HostSync &sync) {
-                         const int f = by % frames, st = by / frames, frame = frame0 + f;
+            run_grid((int)(X * U), 1, (1 << L) / 16, fft::Sizes<(1 << L)>::kLds * sizeof(float2),
+                     [=](int j, int id, int, char *lds_bytes, HostSync &sync) {
+                         unsigned qu, u;
+                         fft::staged_unit((unsigned)id, X, U, qu, u);
+                         const int bx = (int)qu, f = (int)(u % (unsigned)frames), st = (int)(u / (unsigned)frames);
+                         const int frame = frame0 + f;
                          float2 *lds = reinterpret_cast<float2 *>(lds_bytes);
                          auto at = [&](int k) { return forms::spec_index(st, frame, k, n_freq, fcap, tile); };
                          const float *g = gain == nullptr ? nullptr : gain + (size_t)st * n_freq;

@@ -455,6 +455,28 @@ static int check_staged(int n, bool one_kernel) {
     return bad;
 }
 
+// The one-kernel staged form's workgroup mapping is a bijection onto (pair,
+// unit), and the pairs of one unit lie 8 ids apart outside the tail.
+static int check_staged_unit() {
+    int bad = 0;
+    for (unsigned X = 1; X <= 17; ++X)
+        for (unsigned U = 1; U <= 41; ++U) {
+            std::vector<int> seen((size_t)X * U, 0);
+            std::vector<long> first(U, -1);
+            for (unsigned id = 0; id < X * U; ++id) {
+                unsigned q, u;
+                staged_unit(id, X, U, q, u);
+                if (q >= X || u >= U) { ++bad; continue; }
+                ++seen[(size_t)u * X + q];
+                if (first[u] < 0) first[u] = id;
+                else if (u < U / 8 * 8 && (id - (unsigned)first[u]) % 8 != 0) ++bad;
+            }
+            for (int c : seen) bad += c != 1;
+        }
+    std::printf("staged workgroup mapping: %d bad\n", bad);
+    return bad;
+}
+
 // `fengine_emu <part> <parts>` runs every parts-th check from the part-th on, so
 // that the checks can run in parallel processes; no arguments run them all.
 static int g_part = 0, g_parts = 1, g_idx = 0;
@@ -486,6 +508,7 @@ int main(int argc, char **argv) {
     if (mine()) bad += check_staged(40960, false);  // m = 5
     // the one-kernel form: where the engine uses it (m = 2, 3, 4 at P = 16384),
     // and at small P and odd m, where only this check runs it
+    if (mine()) bad += check_staged_unit();
     if (mine()) bad += check_staged(32768, true);
     if (mine()) bad += check_staged(49152, true);  // m = 3: a mirror pair and residue 0
     if (mine()) bad += check_staged(65536, true);  // m = 4: a mirror pair and two residues alone
