@@ -223,8 +223,12 @@ staged_kernel(const int8_t *__restrict__ in, size_t station_pitch, const int32_t
     const int fcap = (int)(out_station_stride / (size_t)n);
     auto at = [&](int k) { return forms::spec_index(st, frame, k, n, fcap, tile); };
     const int j = (int)threadIdx.x;
-    static_assert(FORM == 1 || FORM == 2, "the one-kernel form writes the requantized forms");
-    if constexpr (FORM == 1) {
+    if constexpr (FORM == 0) {
+        float2 *o = static_cast<float2 *>(out);
+        fft::staged_frame<LOG2P>(j, q, f, st, pl, in, station_pitch, frame0, valid, valid_stride, lds, sync, tw,
+                                 tw + n, wl, [&](int k, float2 x) { o[at(k)] = x; },
+                                 [&](int k) { o[at(k)] = make_float2(0.0f, 0.0f); });
+    } else if constexpr (FORM == 1) {
         uint32_t *o = static_cast<uint32_t *>(out);
         const float *g = gain + (size_t)st * (size_t)n;
         fft::staged_frame<LOG2P>(j, q, f, st, pl, in, station_pitch, frame0, valid, valid_stride, lds, sync, tw,
@@ -554,13 +558,29 @@ hipError_t launch_fengine_staged(int form, int n_freq, const int8_t *in, size_t 
     if (fft::staged_one_kernel(form, pl)) {
         const size_t groups = (size_t)(pl.m / 2 + 1) * (size_t)frames * (size_t)n_station;
         if (groups > 0x7fffffffu) return hipErrorInvalidValue;
-        const dim3 grid((unsigned)groups);
-        if (form == 1)
-            hipLaunchKernelGGL((staged_kernel<14, 1>), grid, dim3(1024), 0, s, in, station_pitch, valid, valid_stride,
-                               frame0, frames, n_station, pl.n, pl.m, tw, wl, gain, out, out_station_stride, tile);
-        else
-            hipLaunchKernelGGL((staged_kernel<14, 2>), grid, dim3(1024), 0, s, in, station_pitch, valid, valid_stride,
-                               frame0, frames, n_station, pl.n, pl.m, tw, wl, gain, out, out_station_stride, tile);
+        const dim3 grid((unsigned)groups), block((unsigned)(pl.p / 16));
+        switch (pl.log2p * 4 + form) {
+#define CORR_STAGED1_CASE(L, F)                                                                                      \
+    case L * 4 + F:                                                                                                   \
+        hipLaunchKernelGGL((staged_kernel<L, F>), grid, block, 0, s, in, station_pitch, valid, valid_stride, frame0,   \
+                           frames, n_station, pl.n, pl.m, tw, wl, gain, out, out_station_stride, tile);              \
+        break;
+            CORR_STAGED1_CASE(11, 0)
+            CORR_STAGED1_CASE(11, 1)
+            CORR_STAGED1_CASE(11, 2)
+            CORR_STAGED1_CASE(12, 0)
+            CORR_STAGED1_CASE(12, 1)
+            CORR_STAGED1_CASE(12, 2)
+            CORR_STAGED1_CASE(13, 0)
+            CORR_STAGED1_CASE(13, 1)
+            CORR_STAGED1_CASE(13, 2)
+            CORR_STAGED1_CASE(14, 0)
+            CORR_STAGED1_CASE(14, 1)
+            CORR_STAGED1_CASE(14, 2)
+#undef CORR_STAGED1_CASE
+        default:
+            return hipErrorInvalidValue;
+        }
         return hipGetLastError();
     }
     if ((size_t)frames * (size_t)n_station > 65535u) return hipErrorInvalidValue;  // the two-kernel grids

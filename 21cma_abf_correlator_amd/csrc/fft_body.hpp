@@ -1461,12 +1461,15 @@ CORR_HD inline StagedPlan staged_plan(int n_freq) {
     return s;
 }
 
-// The one-kernel form where it measured faster than the two-kernel one: the
-// FP16 and int8 forms (4 and 2 bytes a bin) at P = 16384 and m <= 4, where a
-// wave's stores span at most 1 KB. The FP32 form's 8-byte bins, and larger m,
-// scatter further and measured slower than the staging round trip.
+// The one-kernel form where it measured faster than the two-kernel one (2048
+// frames of 24 inputs): the FP32 form at P = 16384 and m = 2 or 4 (-35% and
+// -15%; +4% at m = 3, +11-19% at P <= 4096), the FP16 and int8 forms up to
+// m = 11 (-13% at 18432 = 9 x 2048, -4% at 45056 = 11 x 4096, -35% at 32768,
+// -25% at 65536; +33% at 67584 = 33 x 2048). Where it loses, its stores
+// (a workgroup's bins lie m apart) cost more than the staging round trip.
 CORR_HD inline bool staged_one_kernel(int form, const StagedPlan &pl) {
-    return form != 0 && pl.p == 16384 && pl.m <= 4;
+    if (form == 0) return pl.p == 16384 && (pl.m == 2 || pl.m == 4);
+    return pl.m <= 11;
 }
 
 // Workgroup id -> (residue pair q, unit u = st * frames + f) on the one-kernel

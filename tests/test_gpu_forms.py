@@ -411,10 +411,11 @@ def test_fengine_form_rounding(corr, oracle, hip, form, nf):
     clamp(rint(y*g), +-7), for at least 99.9% of values and every value within
     one FP16 ulp / one integer step (the two kernels share the FFT, so the rare
     difference can only come from a 1-ulp FP32 difference at a rounding tie).
-    Pins the rounding mode of the device's FP16 conversion. At 32768, 49152
-    and 65536 (m = 2, 3, 4 sub-transforms of 16384) the forms' spectra come
-    from the one-kernel staged transform and the FP32 form's from the
-    two-kernel one."""
+    Pins the rounding mode of the device's FP16 conversion. Above 16384 the
+    forms' spectra come from the one-kernel staged transform at every length
+    here, the FP32 form's from the two-kernel one at 18432, 24576 and 49152
+    and from the one-kernel one at 32768 and 65536, so each pair of staged
+    paths is compared."""
     N, T = 4, 8
     print(f"F-engine form {form} rounding N={N} nf={nf}", flush=True)
     x = oracle.gen_int8(21 + form, N, T, 2 * nf, 0.3, sigma=16.0, max_delay=4)
